@@ -1,0 +1,168 @@
+"""Seeded synthetic circuits for the benchmark configurations.
+
+A circuit is a list of ``(name, args)`` pairs where ``name`` is the reference's
+operation name without its ``distributed_statevector_`` / ``distributed_densitymatrix_``
+prefix and ``args`` are that function's arguments after the state
+(reference src/distributed_statevector.hpp, src/distributed_densitymatrix.hpp).
+Both the GPU engine and the CPU oracle consume the same list.
+
+Random laws (numpy PCG64, seed given):
+  * Haar U(d): QR of a complex Ginibre matrix with diag(R) phases removed;
+  * angles theta ~ U(0, 2 pi); probabilities ~ U(0, 0.1);
+  * qubits uniform over the register, distinct within one op.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+TWO_PI = 2 * np.pi
+
+
+def haar(d: int, rng: np.random.Generator) -> np.ndarray:
+    z = (rng.normal(size=(d, d)) + 1j * rng.normal(size=(d, d))) / np.sqrt(2)
+    q, r = np.linalg.qr(z)
+    ph = np.diag(r) / np.abs(np.diag(r))
+    return q * ph[None, :]
+
+
+def _qubits(rng, n, k):
+    return [int(x) for x in rng.choice(n, k, replace=False)]
+
+
+def sv_mix(n: int, num_gates: int, seed: int, extended: bool) -> list:
+    """Statevector gate mix.
+
+    Base mix (12-qubit, 200-gate configuration): oneTargGate (Haar 2x2, uniform
+    target); manyCtrlOneTargGate (2 controls, Haar 2x2); swapGate; pauliGadget
+    on 3 qubits with Paulis in {X, Y, Z} and theta ~ U(0, 2pi).
+    Extended mix (30-qubit, 1000-gate configuration) adds manyTargGate with
+    k in {2,3,4,5} (Haar 2^k), phaseGadget on 4 qubits and pauliTensor on 6 qubits.
+    """
+    rng = np.random.default_rng(seed)
+    kinds = ["one", "ctrl", "swap", "gadget"] + (["many", "phase", "tensor"] if extended else [])
+    ops = []
+    for _ in range(num_gates):
+        kind = kinds[int(rng.integers(len(kinds)))]
+        if kind == "one":
+            ops.append(("oneTargGate", (int(rng.integers(n)), haar(2, rng))))
+        elif kind == "ctrl":
+            q = _qubits(rng, n, 3)
+            ops.append(("manyCtrlOneTargGate", (q[1:], q[0], haar(2, rng))))
+        elif kind == "swap":
+            q = _qubits(rng, n, 2)
+            ops.append(("swapGate", (q[0], q[1])))
+        elif kind == "gadget":
+            q = _qubits(rng, n, 3)
+            p = [int(x) for x in rng.integers(1, 4, 3)]
+            ops.append(("pauliGadget", (q, p, float(rng.uniform(0, TWO_PI)))))
+        elif kind == "many":
+            k = int(rng.integers(2, 6))
+            ops.append(("manyTargGate", (_qubits(rng, n, k), haar(1 << k, rng))))
+        elif kind == "phase":
+            ops.append(("phaseGadget", (_qubits(rng, n, 4), float(rng.uniform(0, TWO_PI)))))
+        else:
+            q = _qubits(rng, n, 6)
+            p = [int(x) for x in rng.integers(1, 4, 6)]
+            ops.append(("pauliTensor", (q, p)))
+    return ops
+
+
+def config1(seed: int = 20261203) -> list:
+    """12-qubit statevector, 200 gates, base mix."""
+    return sv_mix(12, 200, seed, extended=False)
+
+
+def config2(seed: int = 20261204, n: int = 30, num_gates: int = 1000) -> list:
+    """30-qubit statevector, 1000 gates, extended mix."""
+    return sv_mix(n, num_gates, seed, extended=True)
+
+
+def dm_mix(N: int, num_ops: int, seed: int) -> list:
+    """Density-matrix sequence: manyTargGate k in {1,2,3} (Haar); one/two-qubit
+    dephasing and depolarising, damping; prob ~ U(0, 0.1)."""
+    rng = np.random.default_rng(seed)
+    ops = []
+    for _ in range(num_ops):
+        kind = int(rng.integers(6))
+        p = float(rng.uniform(0, 0.1))
+        if kind == 0:
+            k = int(rng.integers(1, 4))
+            ops.append(("manyTargGate", (_qubits(rng, N, k), haar(1 << k, rng))))
+        elif kind == 1:
+            ops.append(("oneQubitDephasing", (int(rng.integers(N)), p)))
+        elif kind == 2:
+            q = _qubits(rng, N, 2)
+            ops.append(("twoQubitDephasing", (q[0], q[1], p)))
+        elif kind == 3:
+            ops.append(("oneQubitDepolarising", (int(rng.integers(N)), p)))
+        elif kind == 4:
+            q = _qubits(rng, N, 2)
+            ops.append(("twoQubitDepolarising", (q[0], q[1], p)))
+        else:
+            ops.append(("damping", (int(rng.integers(N)), p)))
+    return ops
+
+
+def config3(seed: int = 20261205, N: int = 14, num_ops: int = 400):
+    """14-qubit density matrix: 400 ops, then 64 Pauli strings (coefficients
+    N(0,1)) for expecPauliString and 4 qubits for partialTrace."""
+    ops = dm_mix(N, num_ops, seed)
+    rng = np.random.default_rng(seed + 1)
+    paulis = rng.integers(0, 4, 64 * N)
+    coeffs = rng.normal(size=64)
+    trace_targets = _qubits(rng, N, 4)
+    return ops, coeffs, paulis, trace_targets
+
+
+def random_mixed_state(N: int, seed: int, rank: int = 8) -> np.ndarray:
+    """rho = sum_i w_i |g_i><g_i|, w ~ Dirichlet(1), g_i normalised Gaussians;
+    returned as the column-major 2N-qubit vector."""
+    rng = np.random.default_rng(seed)
+    D = 1 << N
+    w = rng.dirichlet(np.ones(rank))
+    R = np.zeros((D, D), dtype=complex)
+    for wi in w:
+        g = rng.normal(size=D) + 1j * rng.normal(size=D)
+        g /= np.linalg.norm(g)
+        R += wi * np.outer(g, g.conj())
+    return R.T.reshape(-1).copy()
+
+
+def config4(n: int = 24, layers: int = 20, seed: int = 20261206, num_terms: int = 100):
+    """Layered ZZ + X gadget circuit from |0..0>: per layer a ZZ gadget on each of
+    the n-1 nearest-neighbour pairs, then an X gadget on every qubit; angles
+    ~ U(0, 2pi).  Observable: num_terms Pauli strings of weight 1..4 on random
+    qubits, codes in {X, Y, Z}, coefficients N(0,1) (term-major n codes each)."""
+    rng = np.random.default_rng(seed)
+    ops = []
+    for _ in range(layers):
+        for q in range(n - 1):
+            ops.append(("pauliGadget", ([q, q + 1], [3, 3], float(rng.uniform(0, TWO_PI)))))
+        for q in range(n):
+            ops.append(("pauliGadget", ([q], [1], float(rng.uniform(0, TWO_PI)))))
+    paulis = np.zeros((num_terms, n), dtype=np.uint32)
+    for t in range(num_terms):
+        w = int(rng.integers(1, 5))
+        qs = _qubits(rng, n, w)
+        paulis[t, qs] = rng.integers(1, 4, w)
+    coeffs = rng.normal(size=num_terms)
+    return ops, coeffs, paulis
+
+
+def algorithmic_bytes(name: str, args, nq: int, amp_bytes: int = 16, dm: bool = False) -> int:
+    """Bytes an op must move: 2 x amp size x amplitudes it can change.
+
+    nq is the number of index bits (n for a statevector, 2N for a density matrix).
+    """
+    full = 2 * amp_bytes * (1 << nq)
+    if dm:
+        if name == "oneQubitDephasing":
+            return full // 2
+        if name == "twoQubitDephasing":
+            return full * 3 // 4
+        return full
+    if name == "manyCtrlOneTargGate":
+        return full >> len(args[0])
+    if name == "swapGate":
+        return full // 2
+    return full

@@ -1,0 +1,120 @@
+// common.hpp -- types and device index arithmetic shared by the kernels and
+// the host side of libfsa.
+//
+// Layout in HBM: one contiguous array of interleaved complex amplitudes per
+// state (complex128 for FP64 states, complex64 for FP32 states), qubit q = bit
+// q of the index; density matrices are the column-major 2N-qubit vector.  One
+// FP64 amplitude is exactly one 16-byte global_load_dwordx4.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace fsa {
+
+template <typename R>
+struct alignas(2 * sizeof(R)) cx {
+    R x, y;
+};
+
+template <typename R>
+__host__ __device__ __forceinline__ cx<R> mk(R a, R b) { cx<R> r; r.x = a; r.y = b; return r; }
+
+template <typename R>
+__host__ __device__ __forceinline__ cx<R> cmul(cx<R> a, cx<R> b) {
+    return mk<R>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+template <typename R>
+__host__ __device__ __forceinline__ cx<R> cadd(cx<R> a, cx<R> b) { return mk<R>(a.x + b.x, a.y + b.y); }
+template <typename R>
+__host__ __device__ __forceinline__ cx<R> cscale(cx<R> a, R s) { return mk<R>(a.x * s, a.y * s); }
+// acc + a*b
+template <typename R>
+__host__ __device__ __forceinline__ cx<R> cfma(cx<R> acc, cx<R> a, cx<R> b) {
+    acc.x = acc.x + a.x * b.x - a.y * b.y;
+    acc.y = acc.y + a.x * b.y + a.y * b.x;
+    return acc;
+}
+
+// ---- index arithmetic (reference src/bit_maths.hpp:19-125 as device functions)
+// Indices are 64-bit: one MI355X holds up to 2^34 FP64 amplitudes.
+
+// open a zero at position b (insertBit(x, b, 0), bit_maths.hpp:43-49)
+__host__ __device__ __forceinline__ uint64_t ins0(uint64_t x, unsigned b) {
+    uint64_t low = x & ((1ULL << b) - 1ULL);
+    return ((x >> b) << (b + 1)) | low;
+}
+__host__ __device__ __forceinline__ uint32_t ins0_32(uint32_t x, unsigned b) {
+    uint32_t low = x & ((1u << b) - 1u);
+    return ((x >> b) << (b + 1)) | low;
+}
+// open zeros at every set bit of `mask`, ascending (insertBits, bit_maths.hpp:52-59)
+__host__ __device__ __forceinline__ uint64_t ins0_mask(uint64_t x, uint64_t mask) {
+    while (mask) {
+        unsigned b = (unsigned)__builtin_ctzll(mask);
+        x = ins0(x, b);
+        mask &= mask - 1;
+    }
+    return x;
+}
+__host__ __device__ __forceinline__ uint32_t ins0_mask32(uint32_t x, uint32_t mask) {
+    while (mask) {
+        unsigned b = (unsigned)__builtin_ctz(mask);
+        x = ins0_32(x, b);
+        mask &= mask - 1;
+    }
+    return x;
+}
+// getBitMaskParity (bit_maths.hpp:81-88)
+__host__ __device__ __forceinline__ unsigned par64(uint64_t x) { return (unsigned)__builtin_popcountll(x) & 1u; }
+__host__ __device__ __forceinline__ unsigned par32(uint32_t x) { return (unsigned)__builtin_popcount(x) & 1u; }
+
+// ---- kernel-argument records -------------------------------------------------
+
+template <typename R>
+struct Mat2 { cx<R> m[4]; };                    // row-major 2x2
+
+// Operation kinds executed by the fused tile kernel and the streaming kernels.
+enum OpKind : uint32_t {
+    OP_MAT1 = 1,     // 2x2 on one target, all-ones control masks
+    OP_MATK = 2,     // dense 2^k x 2^k on k targets (matrix in the parameter buffer)
+    OP_SWAP = 3,     // swap two qubits
+    OP_PAULI = 4,    // (this, other) Pauli tensor/gadget pair map
+    OP_PHASE = 5,    // e^{+-i theta} by parity
+    OP_DEPHASE1 = 6,
+    OP_DEPHASE2 = 7,
+    OP_DEPOL1 = 8,
+    OP_DEPOL2 = 9,
+    OP_DAMP = 10,
+};
+
+// One operation inside a fused tile pass.  Positions p[] are bit positions
+// inside the tile (0..M-1); masks named l* are tile-local, masks named g* are
+// global index masks restricted to qubits outside the tile (their parity or
+// all-ones test is fixed for a whole tile).
+struct TileOp {
+    uint32_t kind;
+    uint32_t k;           // MATK: number of targets; DEPOL2: unused
+    uint32_t p[6];        // tile-local positions (targets in caller order for MATK)
+    uint32_t lmask;       // MAT1: local control mask; PAULI: local XY mask; PHASE: local parity mask
+    uint32_t lmask2;      // PAULI: local YZ mask
+    uint64_t gmask;       // MAT1: out-of-tile control mask; PAULI/PHASE: out-of-tile parity mask
+    uint64_t gq;          // channels: global qubit indices packed 4 x 16 bits (q1, q1+N, q2, q2+N)
+    uint32_t mat_off;     // MATK: offset (in complex elements) of the matrix in the parameter buffer
+    uint32_t param;       // adjoint: parameter index of a gadget, or 0xffffffff
+    double f[8];          // MAT1: 2x2 matrix; PAULI: this, other*powI; PHASE: e^{i t}, e^{-i t}; channels: constants
+};
+static_assert(sizeof(TileOp) % 8 == 0, "TileOp must stay 8-byte aligned");
+
+constexpr uint32_t NO_PARAM = 0xffffffffu;
+constexpr int TILE_MAX_BITS = 14;
+
+// Geometry of one fused pass: the sorted tile qubits (global positions).
+struct TileGeom {
+    uint8_t q[TILE_MAX_BITS];
+    uint8_t m;            // number of tile qubits (tile = 2^m amplitudes)
+    uint8_t pad;
+    uint64_t tileMask;    // OR of 1 << q[i]
+};
+
+}  // namespace fsa

@@ -1,0 +1,444 @@
+// engine.cpp -- fused-pass scheduler, parameter ring and op execution.
+#include "engine.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+
+namespace fsa {
+
+[[noreturn]] void fail(int code, const std::string& msg) { throw Error{code, msg}; }
+
+void hip_check(hipError_t e, const char* what) {
+    if (e != hipSuccess) {
+        if (e == hipErrorOutOfMemory) fail(FSA_ERR_OOM, std::string(what) + ": out of device memory");
+        fail(FSA_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+    }
+}
+
+// ---- operation bookkeeping ---------------------------------------------------
+
+void refresh_angle(HOp& op) {
+    if (op.kind == OP_PAULI) {
+        if (op.gadget) {
+            op.th = cd(std::cos(op.theta), 0.0);
+            op.fe = cd(0.0, std::sin(op.theta)) * op.pw;
+        } else {
+            op.th = 0.0;
+            op.fe = cd(op.other_sign, 0.0) * op.pw;
+        }
+    }
+}
+
+void finish_op(HOp& op) {
+    uint64_t m = 0;
+    for (unsigned x : op.q) m |= 1ULL << x;
+    switch (op.kind) {
+    case OP_MAT1: op.nd = m; op.touch = m | op.ctrl; op.tileable = true; break;
+    case OP_MATK: op.nd = m; op.touch = m; op.tileable = op.q.size() <= 5; break;
+    case OP_SWAP: op.nd = m; op.touch = m; op.tileable = true; break;
+    case OP_PAULI: op.nd = op.mxy; op.touch = op.mxy | op.myz; op.tileable = true; break;
+    case OP_PHASE: op.nd = 0; op.touch = op.pmask; op.tileable = true; break;
+    case OP_DEPHASE1:
+    case OP_DEPHASE2: op.nd = 0; op.touch = m; op.tileable = true; break;
+    case OP_DEPOL1:
+    case OP_DEPOL2:
+    case OP_DAMP: op.nd = m; op.touch = m; op.tileable = true; break;
+    }
+}
+
+// ---- scheduler ---------------------------------------------------------------
+
+static unsigned env_u(const char* name, unsigned dflt) {
+    const char* v = std::getenv(name);
+    if (!v || !*v) return dflt;
+    return (unsigned)std::strtoul(v, nullptr, 10);
+}
+
+SchedParams sched_params(unsigned nq, int precision) {
+    unsigned mmax = precision == FSA_FP64 ? (unsigned)tile_bits_max<double>() : (unsigned)tile_bits_max<float>();
+    unsigned m = std::min(env_u("FSA_TILE_BITS", mmax), mmax);
+    unsigned low = std::min(env_u("FSA_TILE_LOW", 5), m);
+    if (nq <= m) { m = nq; low = nq; }
+    return {m, low};
+}
+
+// Greedy pass builder.  An op joins the current pass when (a) its mixed qubits
+// fit in the tile together with those already chosen and (b) it commutes with
+// every op deferred so far in this scan (they either touch disjoint qubits or
+// act diagonally on every shared one), so executing it early is exact.
+std::vector<Stage> schedule(const std::vector<HOp>& ops, unsigned nq, SchedParams sp) {
+    std::vector<Stage> stages;
+    const uint64_t lowMask = sp.low >= 64 ? ~0ULL : ((1ULL << sp.low) - 1ULL);
+    const unsigned freeSlots = sp.m - sp.low;
+    auto fits = [&](const HOp& op) {
+        return op.tileable && (unsigned)__builtin_popcountll(op.nd & ~lowMask) <= freeSlots;
+    };
+    std::vector<int> remaining(ops.size());
+    for (size_t i = 0; i < ops.size(); i++) remaining[i] = (int)i;
+    while (!remaining.empty()) {
+        if (!fits(ops[remaining[0]])) {
+            Stage st;
+            st.tiled = false;
+            st.ops.push_back(remaining[0]);
+            stages.push_back(st);
+            remaining.erase(remaining.begin());
+            continue;
+        }
+        Stage st;
+        uint64_t tile = lowMask, blockedNd = 0, blockedAll = 0;
+        std::vector<int> deferred;
+        for (int i : remaining) {
+            const HOp& op = ops[i];
+            bool take = false;
+            if (fits(op) && !(op.nd & blockedAll) && !(op.touch & blockedNd)) {
+                uint64_t nt = tile | op.nd;
+                if ((unsigned)__builtin_popcountll(nt) <= sp.m) { tile = nt; take = true; }
+            }
+            if (take) {
+                st.ops.push_back(i);
+            } else {
+                deferred.push_back(i);
+                uint64_t nd = fits(op) ? op.nd : op.touch;
+                blockedNd |= nd;
+                blockedAll |= op.touch;
+            }
+        }
+        for (unsigned q = 0; q < nq && (unsigned)__builtin_popcountll(tile) < sp.m; q++) tile |= 1ULL << q;
+        for (unsigned q = 0; q < nq; q++) if ((tile >> q) & 1) st.tq.push_back(q);
+        st.tiled = st.ops.size() > 1;
+        stages.push_back(st);
+        remaining.swap(deferred);
+    }
+    return stages;
+}
+
+// ---- parameter ring ----------------------------------------------------------
+
+ParamRing::~ParamRing() {
+    if (host) (void)hipHostFree(host);
+    if (dev) (void)hipFree(dev);
+}
+
+void* ParamRing::push(const void* data, size_t bytes) {
+    size_t need = (bytes + 255) & ~(size_t)255;
+    if (need > cap) {
+        if (stream) hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+        if (host) hip_check(hipHostFree(host), "hipHostFree");
+        if (dev) hip_check(hipFree(dev), "hipFree");
+        host = nullptr;
+        dev = nullptr;
+        size_t ncap = std::max<size_t>({need, 2 * cap, (size_t)1 << 20});
+        hip_check(hipHostMalloc((void**)&host, ncap, hipHostMallocDefault), "hipHostMalloc(param ring)");
+        hip_check(hipMalloc((void**)&dev, ncap), "hipMalloc(param ring)");
+        cap = ncap;
+        head = 0;
+    }
+    if (head + need > cap) {
+        hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+        head = 0;
+    }
+    std::memcpy(host + head, data, bytes);
+    hip_check(hipMemcpyAsync(dev + head, host + head, bytes, hipMemcpyHostToDevice, stream), "hipMemcpyAsync(params)");
+    void* d = dev + head;
+    head += need;
+    return d;
+}
+
+double* State::workspace(size_t doubles) {
+    if (doubles > wsDoubles) {
+        if (ws) {
+            hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+            hip_check(hipFree(ws), "hipFree");
+            ws = nullptr;
+        }
+        size_t n = std::max<size_t>(doubles, 8192);
+        hip_check(hipMalloc((void**)&ws, n * sizeof(double)), "hipMalloc(workspace)");
+        wsDoubles = n;
+    }
+    return ws;
+}
+
+// ---- kernel timing -------------------------------------------------------------
+
+namespace {
+
+struct ProfRecord {
+    const char* name;
+    hipEvent_t start, stop;
+};
+bool g_prof = false;
+std::vector<ProfRecord> g_records;
+std::vector<hipEvent_t> g_event_pool;
+
+hipEvent_t prof_event() {
+    if (!g_event_pool.empty()) {
+        hipEvent_t e = g_event_pool.back();
+        g_event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e;
+    hip_check(hipEventCreate(&e), "hipEventCreate");
+    return e;
+}
+
+// brackets one launch on `stream` when timing is enabled
+struct ProfScope {
+    ProfRecord rec{nullptr, nullptr, nullptr};
+    hipStream_t stream;
+    ProfScope(const char* name, hipStream_t s) : stream(s) {
+        if (!g_prof) return;
+        rec = ProfRecord{name, prof_event(), prof_event()};
+        hip_check(hipEventRecord(rec.start, stream), "hipEventRecord");
+    }
+    ~ProfScope() {
+        if (!rec.name) return;
+        (void)hipEventRecord(rec.stop, stream);
+        g_records.push_back(rec);
+    }
+};
+
+const char* op_name(OpKind k) {
+    switch (k) {
+    case OP_MAT1: return "mat1";
+    case OP_MATK: return "matk";
+    case OP_SWAP: return "swap";
+    case OP_PAULI: return "pauli";
+    case OP_PHASE: return "phase";
+    case OP_DEPHASE1: return "dephase1";
+    case OP_DEPHASE2: return "dephase2";
+    case OP_DEPOL1: return "depol1";
+    case OP_DEPOL2: return "depol2";
+    case OP_DAMP: return "damp";
+    }
+    return "op";
+}
+
+}  // namespace
+
+void profile_enable(bool on) {
+    for (auto& r : g_records) { g_event_pool.push_back(r.start); g_event_pool.push_back(r.stop); }
+    g_records.clear();
+    g_prof = on;
+}
+
+std::vector<ProfEntry> profile_read() {
+    std::vector<ProfEntry> out;
+    for (auto& r : g_records) {
+        hip_check(hipEventSynchronize(r.stop), "hipEventSynchronize");
+        float ms = 0;
+        hip_check(hipEventElapsedTime(&ms, r.start, r.stop), "hipEventElapsedTime");
+        auto it = std::find_if(out.begin(), out.end(), [&](const ProfEntry& e) { return e.name == r.name; });
+        if (it == out.end()) { out.push_back(ProfEntry{r.name, 0, 0.0}); it = out.end() - 1; }
+        it->count++;
+        it->ms += ms;
+    }
+    return out;
+}
+
+// ---- execution -----------------------------------------------------------------
+
+namespace {
+
+template <typename R>
+cx<R> to_cx(cd v) { return mk<R>((R)v.real(), (R)v.imag()); }
+
+uint32_t chan_code(unsigned q, const int* pos) {
+    return pos[q] >= 0 ? (0x100u | (uint32_t)pos[q]) : (uint32_t)q;
+}
+
+template <typename R>
+TileOp translate(const HOp& op, const int* pos, uint64_t tileMask, std::vector<cx<R>>& mats) {
+    TileOp t;
+    std::memset(&t, 0, sizeof(t));
+    t.kind = op.kind;
+    t.param = op.param < 0 ? NO_PARAM : (uint32_t)op.param;
+    auto local_mask = [&](uint64_t gm) {
+        uint32_t lm = 0;
+        for (uint64_t m = gm & tileMask; m; m &= m - 1) lm |= 1u << pos[__builtin_ctzll(m)];
+        return lm;
+    };
+    switch (op.kind) {
+    case OP_MAT1:
+        t.p[0] = (uint32_t)pos[op.q[0]];
+        t.lmask = local_mask(op.ctrl);
+        t.gmask = op.ctrl & ~tileMask;
+        for (int i = 0; i < 4; i++) { t.f[2 * i] = op.mat[i].real(); t.f[2 * i + 1] = op.mat[i].imag(); }
+        break;
+    case OP_MATK:
+        t.k = (uint32_t)op.q.size();
+        for (size_t i = 0; i < op.q.size(); i++) t.p[i] = (uint32_t)pos[op.q[i]];
+        t.mat_off = (uint32_t)mats.size();
+        for (const cd& v : op.mat) mats.push_back(to_cx<R>(v));
+        break;
+    case OP_SWAP:
+        t.p[0] = (uint32_t)pos[op.q[0]];
+        t.p[1] = (uint32_t)pos[op.q[1]];
+        break;
+    case OP_PAULI:
+        t.lmask = local_mask(op.mxy);
+        t.lmask2 = local_mask(op.myz);
+        t.gmask = op.myz & ~tileMask;
+        t.f[0] = op.th.real(); t.f[1] = op.th.imag();
+        t.f[2] = op.fe.real(); t.f[3] = op.fe.imag();
+        break;
+    case OP_PHASE:
+        t.lmask = local_mask(op.pmask);
+        t.gmask = op.pmask & ~tileMask;
+        t.f[0] = std::cos(op.theta); t.f[1] = std::sin(op.theta);
+        t.f[2] = std::cos(op.theta); t.f[3] = -std::sin(op.theta);
+        break;
+    case OP_DEPHASE1:
+        t.p[0] = chan_code(op.q[0], pos);
+        t.p[1] = chan_code(op.q[1], pos);
+        t.f[0] = op.c[0];
+        break;
+    case OP_DEPHASE2:
+        for (int i = 0; i < 4; i++) t.p[i] = chan_code(op.q[i], pos);
+        t.f[0] = 1.0 + op.c[0];
+        break;
+    case OP_DEPOL1:
+    case OP_DAMP:
+        t.p[0] = (uint32_t)pos[op.q[0]];
+        t.p[1] = (uint32_t)pos[op.q[1]];
+        t.f[0] = op.c[0]; t.f[1] = op.c[1]; t.f[2] = op.c[2];
+        break;
+    case OP_DEPOL2:
+        for (int i = 0; i < 4; i++) t.p[i] = (uint32_t)pos[op.q[i]];
+        t.f[0] = op.c[0]; t.f[1] = op.c[1]; t.f[2] = 1.0 + op.c[2];
+        break;
+    }
+    return t;
+}
+
+template <typename R>
+void run_standalone(State& s, const HOp& op) {
+    ProfScope prof(op_name(op.kind), s.stream);
+    cx<R>* a = static_cast<cx<R>*>(s.amps);
+    const unsigned nq = s.nq;
+    const unsigned N = s.N;
+    switch (op.kind) {
+    case OP_MAT1: {
+        Mat2<R> g;
+        for (int i = 0; i < 4; i++) g.m[i] = to_cx<R>(op.mat[i]);
+        launch_mat1<R>(s.stream, a, nq, op.q[0], op.ctrl, g);
+        break;
+    }
+    case OP_MATK: {
+        const unsigned k = (unsigned)op.q.size();
+        const uint64_t D = 1ULL << k;
+        if (k <= 5) {
+            std::vector<cx<R>> m(D * D);
+            for (uint64_t i = 0; i < D * D; i++) m[i] = to_cx<R>(op.mat[i]);
+            auto* dm = static_cast<const cx<R>*>(s.ring.push(m.data(), m.size() * sizeof(cx<R>)));
+            launch_matk<R>(s.stream, a, nq, op.q.data(), k, dm, nullptr);
+        } else {
+            // transposed matrix followed by the 2^k index offsets
+            size_t matBytes = D * D * sizeof(cx<R>);
+            std::vector<char> blob(matBytes + D * sizeof(uint64_t));
+            cx<R>* mt = reinterpret_cast<cx<R>*>(blob.data());
+            for (uint64_t r = 0; r < D; r++)
+                for (uint64_t c = 0; c < D; c++) mt[c * D + r] = to_cx<R>(op.mat[r * D + c]);
+            uint64_t* offs = reinterpret_cast<uint64_t*>(blob.data() + matBytes);
+            for (uint64_t l = 0; l < D; l++) {
+                uint64_t o = 0;
+                for (unsigned i = 0; i < k; i++) if ((l >> i) & 1) o |= 1ULL << op.q[i];
+                offs[l] = o;
+            }
+            auto* dmt = static_cast<const cx<R>*>(s.ring.push(blob.data(), blob.size()));
+            launch_matk<R>(s.stream, a, nq, op.q.data(), k, nullptr, dmt);
+        }
+        break;
+    }
+    case OP_SWAP: launch_swap<R>(s.stream, a, nq, op.q[0], op.q[1]); break;
+    case OP_PAULI: launch_pauli<R>(s.stream, a, nq, op.mxy, op.myz, to_cx<R>(op.th), to_cx<R>(op.fe)); break;
+    case OP_PHASE:
+        launch_phase<R>(s.stream, a, nq, op.pmask, mk<R>((R)std::cos(op.theta), (R)std::sin(op.theta)),
+                        mk<R>((R)std::cos(op.theta), (R)-std::sin(op.theta)));
+        break;
+    case OP_DEPHASE1: launch_dephase1<R>(s.stream, a, N, op.q[0], (R)op.c[0]); break;
+    case OP_DEPHASE2: launch_dephase2<R>(s.stream, a, N, op.q[0], op.q[2], (R)op.c[0]); break;
+    case OP_DEPOL1: launch_depol1<R>(s.stream, a, N, op.q[0], (R)op.c[0], (R)op.c[1], (R)op.c[2]); break;
+    case OP_DEPOL2:
+        launch_depol2<R>(s.stream, a, N, op.q[0], op.q[1], (R)op.c[0], (R)op.c[1], (R)op.c[2]);
+        break;
+    case OP_DAMP: launch_damp<R>(s.stream, a, N, op.q[0], (R)op.c[0], (R)op.c[1], (R)op.c[2]); break;
+    }
+    hip_check(hipGetLastError(), "kernel launch");
+}
+
+template <typename R>
+void execute_impl(State& s, const std::vector<HOp>& ops, const std::vector<Stage>& stages) {
+    std::vector<TileOp> tops;
+    std::vector<cx<R>> mats;
+    std::vector<size_t> first(stages.size(), 0);
+    std::vector<TileGeom> geoms(stages.size());
+    for (size_t si = 0; si < stages.size(); si++) {
+        const Stage& st = stages[si];
+        if (!st.tiled) continue;
+        int pos[64];
+        for (int& p : pos) p = -1;
+        TileGeom g;
+        std::memset(&g, 0, sizeof(g));
+        g.m = (uint8_t)st.tq.size();
+        for (size_t i = 0; i < st.tq.size(); i++) {
+            pos[st.tq[i]] = (int)i;
+            g.q[i] = (uint8_t)st.tq[i];
+            g.tileMask |= 1ULL << st.tq[i];
+        }
+        geoms[si] = g;
+        first[si] = tops.size();
+        for (int oi : st.ops) tops.push_back(translate<R>(ops[oi], pos, g.tileMask, mats));
+    }
+    const TileOp* dtops = nullptr;
+    const cx<R>* dmats = nullptr;
+    if (!tops.empty()) {
+        size_t opBytes = tops.size() * sizeof(TileOp);
+        size_t matOff = (opBytes + 255) & ~(size_t)255;
+        std::vector<char> blob(matOff + mats.size() * sizeof(cx<R>));
+        std::memcpy(blob.data(), tops.data(), opBytes);
+        if (!mats.empty()) std::memcpy(blob.data() + matOff, mats.data(), mats.size() * sizeof(cx<R>));
+        char* d = static_cast<char*>(s.ring.push(blob.data(), blob.size()));
+        dtops = reinterpret_cast<const TileOp*>(d);
+        dmats = reinterpret_cast<const cx<R>*>(d + matOff);
+    }
+    for (size_t si = 0; si < stages.size(); si++) {
+        const Stage& st = stages[si];
+        if (st.tiled) {
+            ProfScope prof("tile_pass", s.stream);
+            launch_tile<R>(s.stream, static_cast<cx<R>*>(s.amps), s.nq, geoms[si], dtops + first[si],
+                           (unsigned)st.ops.size(), dmats);
+            hip_check(hipGetLastError(), "tile kernel launch");
+        } else {
+            for (int oi : st.ops) run_standalone<R>(s, ops[oi]);
+        }
+    }
+}
+
+}  // namespace
+
+void execute(State& s, const std::vector<HOp>& ops, const std::vector<Stage>& stages) {
+    if (s.prec == FSA_FP64) execute_impl<double>(s, ops, stages);
+    else execute_impl<float>(s, ops, stages);
+}
+
+void execute_now(State& s, const std::vector<HOp>& ops) {
+    if (ops.size() == 1) {
+        Stage st;
+        st.tiled = false;
+        st.ops.push_back(0);
+        execute(s, ops, {st});
+        return;
+    }
+    execute(s, ops, schedule(ops, s.nq, sched_params(s.nq, s.prec)));
+}
+
+void flush(State& s) {
+    if (s.pending.empty()) return;
+    std::vector<HOp> ops;
+    ops.swap(s.pending);
+    execute(s, ops, schedule(ops, s.nq, sched_params(s.nq, s.prec)));
+}
+
+}  // namespace fsa

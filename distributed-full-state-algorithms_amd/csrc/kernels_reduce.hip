@@ -1,0 +1,360 @@
+// kernels_reduce.hip -- reductions (FP64 accumulation, fixed order), partial
+// trace, Pauli-sum application, state initialisation and precision conversion.
+//
+// Reductions are two-level and deterministic: a fixed grid writes one partial
+// per workgroup (wavefront shuffle, then LDS across the 4 waves), and a single
+// workgroup sums the partials in index order.  Results never depend on timing.
+#include "launch.hpp"
+
+namespace fsa {
+
+namespace {
+
+constexpr int RED_ITEMS = 8;
+constexpr unsigned RED_MAX_BLOCKS = 2048;
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    return v;
+}
+
+// sum of (vr, vi) over the workgroup into lane 0 of wave 0
+__device__ __forceinline__ void block_sum2(double& vr, double& vi) {
+    __shared__ double sr[BLOCK / 64], si[BLOCK / 64];
+    vr = wave_sum(vr);
+    vi = wave_sum(vi);
+    const unsigned lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    if (lane == 0) { sr[wave] = vr; si[wave] = vi; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double a = 0, b = 0;
+        for (int w = 0; w < BLOCK / 64; w++) { a += sr[w]; b += si[w]; }
+        vr = a; vi = b;
+    }
+}
+
+unsigned red_blocks(uint64_t num) {
+    uint64_t per = (uint64_t)BLOCK * RED_ITEMS;
+    uint64_t b = (num + per - 1) / per;
+    if (b < 1) b = 1;
+    if (b > RED_MAX_BLOCKS) b = RED_MAX_BLOCKS;
+    return (unsigned)b;
+}
+
+__global__ __launch_bounds__(BLOCK) void k_final_sum(const double* __restrict__ part, unsigned numParts,
+                                                     double* __restrict__ out) {
+    double vr = 0, vi = 0;
+    for (unsigned i = threadIdx.x; i < numParts; i += BLOCK) { vr += part[2 * i]; vi += part[2 * i + 1]; }
+    block_sum2(vr, vi);
+    if (threadIdx.x == 0) { out[0] = vr; out[1] = vi; }
+}
+
+template <typename R>
+__global__ __launch_bounds__(BLOCK) void k_norm2(const cx<R>* __restrict__ a, uint64_t num,
+                                                 double* __restrict__ part) {
+    double acc = 0;
+    for (uint64_t j = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; j < num; j += (uint64_t)gridDim.x * BLOCK) {
+        cx<R> v = a[j];
+        acc += (double)v.x * (double)v.x + (double)v.y * (double)v.y;
+    }
+    double dummy = 0;
+    block_sum2(acc, dummy);
+    if (threadIdx.x == 0) { part[2 * blockIdx.x] = acc; part[2 * blockIdx.x + 1] = 0; }
+}
+
+template <typename R>
+__global__ __launch_bounds__(BLOCK) void k_inner(const cx<R>* __restrict__ a, const cx<R>* __restrict__ b,
+                                                 uint64_t num, double* __restrict__ part) {
+    double vr = 0, vi = 0;
+    for (uint64_t j = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; j < num; j += (uint64_t)gridDim.x * BLOCK) {
+        cx<R> x = a[j], y = b[j];
+        // conj(x) * y
+        vr += (double)x.x * (double)y.x + (double)x.y * (double)y.y;
+        vi += (double)x.x * (double)y.y - (double)x.y * (double)y.x;
+    }
+    block_sum2(vr, vi);
+    if (threadIdx.x == 0) { part[2 * blockIdx.x] = vr; part[2 * blockIdx.x + 1] = vi; }
+}
+
+// Pauli-string sums.  Workgroup (t, c) handles term t over chunk c of 2^N (DM)
+// or 2^n (SV) indices and writes f_t * (its partial sum).
+//  DM (reference distributed_densitymatrix.hpp:322-344 / misc.hpp:16-38): only
+//  entries with column c = r ^ mxy are nonzero, each sigma product equals
+//  (-i)^{#Y} (-1)^{par(c & myz)}, so Tr(P rho) = (-i)^{#Y} sum_r (-1)^{..} rho[r + 2^N c].
+//  SV: <psi|P|psi> = i^{#Y} sum_j conj(psi_j) (-1)^{par(j1 & myz)} psi_{j1}, j1 = j ^ mxy.
+constexpr uint64_t EXP_CHUNK = (uint64_t)BLOCK * 16;
+
+template <typename R, bool DM>
+__global__ __launch_bounds__(BLOCK) void k_pauli_expec(const cx<R>* __restrict__ a, unsigned nbits,
+                                                       const PauliTermDev* __restrict__ terms,
+                                                       unsigned chunks, double* __restrict__ part) {
+    const unsigned t = blockIdx.x / chunks, c = blockIdx.x % chunks;
+    const PauliTermDev term = terms[t];
+    const uint64_t num = 1ULL << nbits;
+    const uint64_t start = (uint64_t)c * EXP_CHUNK;
+    const uint64_t end = start + EXP_CHUNK < num ? start + EXP_CHUNK : num;
+    double vr = 0, vi = 0;
+    for (uint64_t r = start + threadIdx.x; r < end; r += BLOCK) {
+        uint64_t r1 = r ^ term.mxy;
+        double sg = par64(r1 & term.myz) ? -1.0 : 1.0;
+        if (DM) {
+            cx<R> v = a[r + (r1 << nbits)];
+            vr += sg * (double)v.x;
+            vi += sg * (double)v.y;
+        } else {
+            cx<R> x = a[r], y = a[r1];
+            vr += sg * ((double)x.x * (double)y.x + (double)x.y * (double)y.y);
+            vi += sg * ((double)x.x * (double)y.y - (double)x.y * (double)y.x);
+        }
+    }
+    block_sum2(vr, vi);
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = term.fr * vr - term.fi * vi;
+        part[2 * blockIdx.x + 1] = term.fr * vi + term.fi * vr;
+    }
+}
+
+// <a| f P |b> = f sum_j conj(a_j) (-1)^{par(j1 & myz)} b_{j1}, j1 = j ^ mxy; one partial per workgroup
+template <typename R>
+__global__ __launch_bounds__(BLOCK) void k_pauli_element(const cx<R>* __restrict__ a, const cx<R>* __restrict__ b,
+                                                         uint64_t num, PauliTermDev term,
+                                                         double* __restrict__ part) {
+    double vr = 0, vi = 0;
+    for (uint64_t j = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; j < num; j += (uint64_t)gridDim.x * BLOCK) {
+        uint64_t j1 = j ^ term.mxy;
+        double sg = par64(j1 & term.myz) ? -1.0 : 1.0;
+        cx<R> x = a[j], y = b[j1];
+        vr += sg * ((double)x.x * (double)y.x + (double)x.y * (double)y.y);
+        vi += sg * ((double)x.x * (double)y.y - (double)x.y * (double)y.x);
+    }
+    block_sum2(vr, vi);
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = term.fr * vr - term.fi * vi;
+        part[2 * blockIdx.x + 1] = term.fr * vi + term.fi * vr;
+    }
+}
+
+// out[j] = sum_t f_t (-1)^{par(j1 & myz_t)} in[j1], j1 = j ^ mxy_t
+template <typename R>
+__global__ __launch_bounds__(BLOCK) void k_apply_pauli_sum(const cx<R>* __restrict__ in, cx<R>* __restrict__ out,
+                                                           uint64_t num, const PauliTermDev* __restrict__ terms,
+                                                           unsigned numTerms) {
+    const uint64_t j = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (j >= num) return;
+    double ar = 0, ai = 0;
+    for (unsigned t = 0; t < numTerms; t++) {
+        const PauliTermDev term = terms[t];
+        uint64_t j1 = j ^ term.mxy;
+        double sg = par64(j1 & term.myz) ? -1.0 : 1.0;
+        cx<R> v = in[j1];
+        double fr = sg * term.fr, fi = sg * term.fi;
+        ar += fr * (double)v.x - fi * (double)v.y;
+        ai += fr * (double)v.y + fi * (double)v.x;
+    }
+    out[j] = mk<R>((R)ar, (R)ai);
+}
+
+struct TargetList { uint32_t t[32]; uint32_t k; };
+
+// reference local_densitymatrix.hpp:134-164 (setBit defect corrected)
+template <typename R>
+__global__ __launch_bounds__(BLOCK) void k_partial_trace(const cx<R>* __restrict__ in, cx<R>* __restrict__ out,
+                                                         uint64_t numOut, unsigned N, uint64_t allMask,
+                                                         TargetList tl) {
+    const uint64_t l = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (l >= numOut) return;
+    const uint64_t lm = ins0_mask(l, allMask);
+    cx<R> acc = mk<R>(0, 0);
+    const uint64_t numTraced = 1ULL << tl.k;
+    for (uint64_t kk = 0; kk < numTraced; kk++) {
+        uint64_t i = lm;
+        for (unsigned b = 0; b < tl.k; b++)
+            if ((kk >> b) & 1) i |= (1ULL << tl.t[b]) | (1ULL << (tl.t[b] + N));
+        acc = cadd(acc, in[i]);
+    }
+    out[l] = acc;
+}
+
+// ---- initialisation ----------------------------------------------------------
+
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+    z += 0x9E3779B97F4A7C15ULL;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+
+__device__ __forceinline__ double unit_open(uint64_t key, uint64_t c) {
+    return ((double)(mix64(key + c) >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+}
+
+template <typename R>
+__global__ __launch_bounds__(BLOCK) void k_gaussian(cx<R>* __restrict__ a, uint64_t num, uint64_t key) {
+    for (uint64_t j = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; j < num; j += (uint64_t)gridDim.x * BLOCK) {
+        double u1 = unit_open(key, 2 * j), u2 = unit_open(key, 2 * j + 1);
+        double r = sqrt(-2.0 * log(u1));
+        double ph = 6.283185307179586476925286766559 * u2;
+        a[j] = mk<R>((R)(r * cos(ph)), (R)(r * sin(ph)));
+    }
+}
+
+template <typename R>
+__global__ __launch_bounds__(BLOCK) void k_scale_inv_sqrt(cx<R>* __restrict__ a, uint64_t num,
+                                                          const double* __restrict__ norm2) {
+    const double f = 1.0 / sqrt(norm2[0]);
+    for (uint64_t j = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; j < num; j += (uint64_t)gridDim.x * BLOCK) {
+        cx<R> v = a[j];
+        a[j] = mk<R>((R)((double)v.x * f), (R)((double)v.y * f));
+    }
+}
+
+template <typename R>
+__global__ __launch_bounds__(BLOCK) void k_basis(cx<R>* __restrict__ a, uint64_t num, uint64_t index) {
+    for (uint64_t j = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; j < num; j += (uint64_t)gridDim.x * BLOCK)
+        a[j] = mk<R>((R)(j == index ? 1 : 0), (R)0);
+}
+
+__global__ __launch_bounds__(BLOCK) void k_f64_to_f32(const cx<double>* __restrict__ in, cx<float>* __restrict__ out,
+                                                      uint64_t num) {
+    for (uint64_t j = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; j < num; j += (uint64_t)gridDim.x * BLOCK)
+        out[j] = mk<float>((float)in[j].x, (float)in[j].y);
+}
+
+__global__ __launch_bounds__(BLOCK) void k_f32_to_f64(const cx<float>* __restrict__ in, cx<double>* __restrict__ out,
+                                                      uint64_t num) {
+    for (uint64_t j = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; j < num; j += (uint64_t)gridDim.x * BLOCK)
+        out[j] = mk<double>((double)in[j].x, (double)in[j].y);
+}
+
+unsigned grid_stride_blocks(uint64_t num) {
+    uint64_t b = (num + BLOCK - 1) / BLOCK;
+    if (b > 65536) b = 65536;
+    return (unsigned)(b ? b : 1);
+}
+
+}  // namespace
+
+uint64_t reduce_workspace_doubles(uint64_t num) {
+    (void)num;
+    return 2ULL * RED_MAX_BLOCKS;
+}
+
+template <typename R>
+void launch_norm2(hipStream_t s, const cx<R>* a, uint64_t num, double* ws, double* dout) {
+    unsigned nb = red_blocks(num);
+    hipLaunchKernelGGL(k_norm2<R>, dim3(nb), dim3(BLOCK), 0, s, a, num, ws);
+    hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(BLOCK), 0, s, ws, nb, dout);
+}
+
+template <typename R>
+void launch_inner(hipStream_t s, const cx<R>* a, const cx<R>* b, uint64_t num, double* ws, double* dout) {
+    unsigned nb = red_blocks(num);
+    hipLaunchKernelGGL(k_inner<R>, dim3(nb), dim3(BLOCK), 0, s, a, b, num, ws);
+    hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(BLOCK), 0, s, ws, nb, dout);
+}
+
+template <typename R>
+void launch_pauli_element(hipStream_t s, const cx<R>* a, const cx<R>* b, uint64_t num, PauliTermDev term,
+                          double* ws, double* dout) {
+    unsigned nb = red_blocks(num);
+    hipLaunchKernelGGL(k_pauli_element<R>, dim3(nb), dim3(BLOCK), 0, s, a, b, num, term, ws);
+    hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(BLOCK), 0, s, ws, nb, dout);
+}
+
+// ws must hold 2 * numTerms * ceil(2^nbits / EXP_CHUNK) doubles
+template <typename R>
+void launch_dm_expec(hipStream_t s, const cx<R>* rho, unsigned N, const PauliTermDev* dterms,
+                     unsigned numTerms, double* ws, double* dout) {
+    unsigned chunks = (unsigned)(((1ULL << N) + EXP_CHUNK - 1) / EXP_CHUNK);
+    unsigned nb = numTerms * chunks;
+    hipLaunchKernelGGL((k_pauli_expec<R, true>), dim3(nb), dim3(BLOCK), 0, s, rho, N, dterms, chunks, ws);
+    hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(BLOCK), 0, s, ws, nb, dout);
+}
+
+template <typename R>
+void launch_sv_expec(hipStream_t s, const cx<R>* psi, unsigned n, const PauliTermDev* dterms,
+                     unsigned numTerms, double* ws, double* dout) {
+    unsigned chunks = (unsigned)(((1ULL << n) + EXP_CHUNK - 1) / EXP_CHUNK);
+    unsigned nb = numTerms * chunks;
+    hipLaunchKernelGGL((k_pauli_expec<R, false>), dim3(nb), dim3(BLOCK), 0, s, psi, n, dterms, chunks, ws);
+    hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(BLOCK), 0, s, ws, nb, dout);
+}
+
+uint64_t expec_workspace_doubles(unsigned nbits, unsigned numTerms) {
+    return 2ULL * numTerms * ((((1ULL << nbits) + EXP_CHUNK - 1) / EXP_CHUNK));
+}
+
+template <typename R>
+void launch_apply_pauli_sum(hipStream_t s, const cx<R>* in, cx<R>* out, unsigned n, const PauliTermDev* dterms,
+                            unsigned numTerms) {
+    uint64_t num = 1ULL << n;
+    hipLaunchKernelGGL(k_apply_pauli_sum<R>, dim3((unsigned)((num + BLOCK - 1) / BLOCK)), dim3(BLOCK), 0, s, in,
+                       out, num, dterms, numTerms);
+}
+
+template <typename R>
+void launch_partial_trace(hipStream_t s, const cx<R>* in, cx<R>* out, unsigned N, const unsigned* sortedTargets,
+                          unsigned k) {
+    TargetList tl{};
+    uint64_t allMask = 0;
+    for (unsigned i = 0; i < k; i++) {
+        tl.t[i] = sortedTargets[i];
+        allMask |= (1ULL << sortedTargets[i]) | (1ULL << (sortedTargets[i] + N));
+    }
+    tl.k = k;
+    uint64_t numOut = 1ULL << (2 * (N - k));
+    hipLaunchKernelGGL(k_partial_trace<R>, dim3((unsigned)((numOut + BLOCK - 1) / BLOCK)), dim3(BLOCK), 0, s, in,
+                       out, numOut, N, allMask, tl);
+}
+
+template <typename R>
+void launch_init_gaussian(hipStream_t s, cx<R>* a, unsigned nq, uint64_t seed) {
+    // the host mixes the seed the same way the oracle does (key = mix64(seed))
+    uint64_t z = seed + 0x9E3779B97F4A7C15ULL;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    uint64_t key = z ^ (z >> 31);
+    uint64_t num = 1ULL << nq;
+    hipLaunchKernelGGL(k_gaussian<R>, dim3(grid_stride_blocks(num)), dim3(BLOCK), 0, s, a, num, key);
+}
+
+template <typename R>
+void launch_scale_by_inv_sqrt(hipStream_t s, cx<R>* a, uint64_t num, const double* dnorm2) {
+    hipLaunchKernelGGL(k_scale_inv_sqrt<R>, dim3(grid_stride_blocks(num)), dim3(BLOCK), 0, s, a, num, dnorm2);
+}
+
+template <typename R>
+void launch_init_basis(hipStream_t s, cx<R>* a, uint64_t num, uint64_t index) {
+    hipLaunchKernelGGL(k_basis<R>, dim3(grid_stride_blocks(num)), dim3(BLOCK), 0, s, a, num, index);
+}
+
+void launch_f64_to_f32(hipStream_t s, const cx<double>* in, cx<float>* out, uint64_t num) {
+    hipLaunchKernelGGL(k_f64_to_f32, dim3(grid_stride_blocks(num)), dim3(BLOCK), 0, s, in, out, num);
+}
+
+void launch_f32_to_f64(hipStream_t s, const cx<float>* in, cx<double>* out, uint64_t num) {
+    hipLaunchKernelGGL(k_f32_to_f64, dim3(grid_stride_blocks(num)), dim3(BLOCK), 0, s, in, out, num);
+}
+
+#define FSA_INSTANTIATE_RED(R)                                                                               \
+    template void launch_norm2<R>(hipStream_t, const cx<R>*, uint64_t, double*, double*);                     \
+    template void launch_inner<R>(hipStream_t, const cx<R>*, const cx<R>*, uint64_t, double*, double*);       \
+    template void launch_pauli_element<R>(hipStream_t, const cx<R>*, const cx<R>*, uint64_t, PauliTermDev,     \
+                                          double*, double*);                                                  \
+    template void launch_dm_expec<R>(hipStream_t, const cx<R>*, unsigned, const PauliTermDev*, unsigned,       \
+                                     double*, double*);                                                       \
+    template void launch_sv_expec<R>(hipStream_t, const cx<R>*, unsigned, const PauliTermDev*, unsigned,       \
+                                     double*, double*);                                                       \
+    template void launch_apply_pauli_sum<R>(hipStream_t, const cx<R>*, cx<R>*, unsigned, const PauliTermDev*, \
+                                            unsigned);                                                        \
+    template void launch_partial_trace<R>(hipStream_t, const cx<R>*, cx<R>*, unsigned, const unsigned*,       \
+                                          unsigned);                                                          \
+    template void launch_init_gaussian<R>(hipStream_t, cx<R>*, unsigned, uint64_t);                           \
+    template void launch_scale_by_inv_sqrt<R>(hipStream_t, cx<R>*, uint64_t, const double*);                  \
+    template void launch_init_basis<R>(hipStream_t, cx<R>*, uint64_t, uint64_t);
+
+FSA_INSTANTIATE_RED(double)
+FSA_INSTANTIATE_RED(float)
+
+}  // namespace fsa

@@ -1,0 +1,82 @@
+// launch.hpp -- host-side launchers of the HIP kernels (internal to libfsa).
+#pragma once
+
+#include "common.hpp"
+
+namespace fsa {
+
+constexpr int BLOCK = 256;
+
+template <typename R> struct Offs32 { uint64_t o[32]; };
+
+// ---- streaming kernels: one HBM pass per operation ---------------------------
+template <typename R>
+void launch_mat1(hipStream_t s, cx<R>* a, unsigned nq, unsigned t, uint64_t ctrlMask, const Mat2<R>& g);
+template <typename R>
+void launch_swap(hipStream_t s, cx<R>* a, unsigned nq, unsigned q1, unsigned q2);
+// dense k-target gate, matrix (device, row-major) dmat; k <= 12
+template <typename R>
+void launch_matk(hipStream_t s, cx<R>* a, unsigned nq, const unsigned* targets, unsigned k,
+                 const cx<R>* dmat, const cx<R>* dmatT);
+template <typename R>
+void launch_pauli(hipStream_t s, cx<R>* a, unsigned nq, uint64_t mxy, uint64_t myz, cx<R> thisFac,
+                  cx<R> evenFac);
+template <typename R>
+void launch_phase(hipStream_t s, cx<R>* a, unsigned nq, uint64_t mask, cx<R> f0, cx<R> f1);
+template <typename R>
+void launch_dephase1(hipStream_t s, cx<R>* a, unsigned N, unsigned q, R fac);
+template <typename R>
+void launch_dephase2(hipStream_t s, cx<R>* a, unsigned N, unsigned q1, unsigned q2, R term);
+template <typename R>
+void launch_depol1(hipStream_t s, cx<R>* a, unsigned N, unsigned q, R c1, R c2, R c3);
+template <typename R>
+void launch_depol2(hipStream_t s, cx<R>* a, unsigned N, unsigned q1, unsigned q2, R c1, R c2, R c3);
+template <typename R>
+void launch_damp(hipStream_t s, cx<R>* a, unsigned N, unsigned q, R prob, R c1, R c2);
+template <typename R>
+void launch_scale(hipStream_t s, cx<R>* a, uint64_t num, R f);
+
+// ---- fused tile pass ----------------------------------------------------------
+template <typename R>
+void launch_tile(hipStream_t s, cx<R>* a, unsigned nq, const TileGeom& geom, const TileOp* dops,
+                 unsigned numOps, const cx<R>* dmats);
+// largest tile (log2 amplitudes) the LDS budget allows for this precision
+template <typename R> constexpr int tile_bits_max() { return sizeof(R) == 8 ? 12 : 13; }
+
+// ---- init / copy / conversion ------------------------------------------------
+template <typename R>
+void launch_init_gaussian(hipStream_t s, cx<R>* a, unsigned nq, uint64_t seed);
+template <typename R>
+void launch_init_basis(hipStream_t s, cx<R>* a, uint64_t num, uint64_t index);
+template <typename R>
+void launch_scale_by_inv_sqrt(hipStream_t s, cx<R>* a, uint64_t num, const double* dnorm2);
+void launch_f64_to_f32(hipStream_t s, const cx<double>* in, cx<float>* out, uint64_t num);
+void launch_f32_to_f64(hipStream_t s, const cx<float>* in, cx<double>* out, uint64_t num);
+
+// ---- reductions (FP64 accumulation, fixed order: deterministic) --------------
+// partial-sum workspace requirement in doubles for a reduction over `num` items
+uint64_t reduce_workspace_doubles(uint64_t num);
+template <typename R>
+void launch_norm2(hipStream_t s, const cx<R>* a, uint64_t num, double* ws, double* dout);
+template <typename R>
+void launch_inner(hipStream_t s, const cx<R>* a, const cx<R>* b, uint64_t num, double* ws, double* dout);
+// sum over terms of coeff_t * (-i)^{nY_t} * sum_r sign * rho[r + 2^N (r ^ mxy_t)]
+struct PauliTermDev { uint64_t mxy, myz; double fr, fi; };   // factor f = coeff * phase
+uint64_t expec_workspace_doubles(unsigned nbits, unsigned numTerms);
+template <typename R>
+void launch_pauli_element(hipStream_t s, const cx<R>* a, const cx<R>* b, uint64_t num, PauliTermDev term,
+                          double* ws, double* dout);
+template <typename R>
+void launch_dm_expec(hipStream_t s, const cx<R>* rho, unsigned N, const PauliTermDev* dterms,
+                     unsigned numTerms, double* ws, double* dout);
+template <typename R>
+void launch_sv_expec(hipStream_t s, const cx<R>* psi, unsigned n, const PauliTermDev* dterms,
+                     unsigned numTerms, double* ws, double* dout);
+template <typename R>
+void launch_apply_pauli_sum(hipStream_t s, const cx<R>* in, cx<R>* out, unsigned n,
+                            const PauliTermDev* dterms, unsigned numTerms);
+template <typename R>
+void launch_partial_trace(hipStream_t s, const cx<R>* in, cx<R>* out, unsigned N,
+                          const unsigned* sortedTargets, unsigned k);
+
+}  // namespace fsa
