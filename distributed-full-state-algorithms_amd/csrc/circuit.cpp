@@ -40,6 +40,8 @@ void add(Circuit& c, HOp op) {
     c.cachedPrec = -1;
 }
 
+void invalidate(Circuit& c) { c.cachedPrec = -1; }
+
 std::vector<cd> mat_of(const double* g, uint64_t dim) {
     need(g != nullptr, FSA_ERR_ARG, "null gate matrix");
     std::vector<cd> m(dim * dim);
@@ -64,9 +66,9 @@ HOp pauli_op(const unsigned* t, const unsigned* p, unsigned num, bool gadget, do
     return o;
 }
 
-const std::vector<Stage>& schedule_for(Circuit& c, unsigned nq, int prec) {
+const Plan& plan_for(Circuit& c, unsigned nq, int prec) {
     if (c.cachedNq != nq || c.cachedPrec != prec) {
-        c.cached = schedule(c.ops, nq, sched_params(nq, prec));
+        c.cached = make_plan(c.ops, nq, prec);
         c.cachedNq = nq;
         c.cachedPrec = prec;
     }
@@ -120,7 +122,7 @@ void adjoint_impl(Circuit& c, State& psi0, const double* coeffs, unsigned T, con
     Scratch phi(n, psi0.prec, st), lam(n, psi0.prec, st);
     hip_check(hipMemcpyAsync(phi.s.amps, psi0.amps, psi0.num() * psi0.amp_bytes(), hipMemcpyDeviceToDevice, st),
               "hipMemcpyAsync");
-    execute(phi.s, c.ops, schedule_for(c, n, psi0.prec));
+    execute(phi.s, plan_for(c, n, psi0.prec));
 
     std::vector<PauliTermDev> terms(T);
     for (unsigned t = 0; t < T; t++) {
@@ -301,6 +303,11 @@ int fsa_circuit_set_params(fsa_circuit* h, const double* thetas, unsigned numPar
                 o.theta = thetas[o.param];
                 refresh_angle(o);
             }
+        for (HOp& o : c.cached.ops)
+            if (o.param >= 0) {
+                o.theta = thetas[o.param];
+                refresh_angle(o);
+            }
     });
 }
 
@@ -312,7 +319,7 @@ int fsa_circuit_apply(fsa_circuit* h, fsa_state* psi) {
         need(!s.dm, FSA_ERR_KIND, "circuits act on statevectors");
         need(c.maxQubit <= s.N, FSA_ERR_QUBIT, "circuit uses a qubit the state does not have");
         flush(s);
-        execute(s, c.ops, schedule_for(c, s.nq, s.prec));
+        execute(s, plan_for(c, s.nq, s.prec));
     });
 }
 
@@ -322,7 +329,7 @@ int fsa_circuit_num_passes(fsa_circuit* h, unsigned numQubits, int precision, un
         need(out != nullptr, FSA_ERR_ARG, "null output");
         need(precision == FSA_FP64 || precision == FSA_FP32, FSA_ERR_ARG, "bad precision");
         need(c.maxQubit <= numQubits, FSA_ERR_QUBIT, "circuit uses a qubit the state does not have");
-        const auto& stages = schedule_for(c, numQubits, precision);
+        const auto& stages = plan_for(c, numQubits, precision).stages;
         unsigned n = 0;
         for (const Stage& st : stages) n += st.tiled ? 1 : (unsigned)st.ops.size();
         *out = n;

@@ -86,21 +86,27 @@ enum OpKind : uint32_t {
     OP_DEPOL1 = 8,
     OP_DEPOL2 = 9,
     OP_DAMP = 10,
+    OP_SWEEP = 11,   // tile pass item: a run of ops applied to register groups (see kernels_tile.hip)
 };
 
-// One operation inside a fused tile pass.  Positions p[] are bit positions
-// inside the tile (0..M-1); masks named l* are tile-local, masks named g* are
-// global index masks restricted to qubits outside the tile (their parity or
-// all-ones test is fixed for a whole tile).
+// One item of a fused tile pass, or one micro-op inside a sweep item.
+// Item form: positions p[] are bit positions inside the tile (0..m-1); masks
+// named l* are tile-local, gmask is a global index mask restricted to qubits
+// outside the tile (its parity or all-ones test is fixed for a whole tile).
+// OP_SWEEP items: p[0..W-1] = the sweep's tile bits (ascending), k = number of
+// micro-ops, mat_off = index of the first micro-op.
+// Micro-op form (inside a sweep): p[] and l* masks are sweep-local bit
+// indices (0..W-1), gq is a tile-local mask of bits outside the sweep (fixed
+// per register group), gmask as above.
 struct TileOp {
     uint32_t kind;
-    uint32_t k;           // MATK: number of targets; DEPOL2: unused
-    uint32_t p[6];        // tile-local positions (targets in caller order for MATK)
-    uint32_t lmask;       // MAT1: local control mask; PAULI: local XY mask; PHASE: local parity mask
-    uint32_t lmask2;      // PAULI: local YZ mask
+    uint32_t k;           // MATK: number of targets; SWEEP: number of micro-ops
+    uint32_t p[6];        // positions (targets in caller order for item-form MATK, ascending in micro form)
+    uint32_t lmask;       // MAT1: control mask; PAULI: XY mask; PHASE: parity mask
+    uint32_t lmask2;      // PAULI: YZ mask
     uint64_t gmask;       // MAT1: out-of-tile control mask; PAULI/PHASE: out-of-tile parity mask
-    uint64_t gq;          // channels: global qubit indices packed 4 x 16 bits (q1, q1+N, q2, q2+N)
-    uint32_t mat_off;     // MATK: offset (in complex elements) of the matrix in the parameter buffer
+    uint64_t gq;          // micro form: the same masks restricted to tile bits outside the sweep
+    uint32_t mat_off;     // MATK: matrix offset (complex elements); SWEEP: first micro-op
     uint32_t param;       // adjoint: parameter index of a gadget, or 0xffffffff
     double f[8];          // MAT1: 2x2 matrix; PAULI: this, other*powI; PHASE: e^{i t}, e^{-i t}; channels: constants
 };

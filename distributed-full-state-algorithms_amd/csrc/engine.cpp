@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
+#include <cstdio>
 #include <cstring>
 
 namespace fsa {
@@ -64,6 +65,23 @@ SchedParams sched_params(unsigned nq, int precision) {
     return {m, low};
 }
 
+// one line per pass: "T|S <tile qubits> : kind[/k] ..." (diagnostics only)
+static void dump_schedule(const char* path, const std::vector<HOp>& ops, const std::vector<Stage>& stages) {
+    FILE* f = std::fopen(path, "a");
+    if (!f) return;
+    for (const Stage& st : stages) {
+        std::fprintf(f, "%s", st.tiled ? "T" : "S");
+        for (unsigned q : st.tq) std::fprintf(f, " %u", q);
+        std::fprintf(f, " :");
+        for (int i : st.ops) {
+            const HOp& o = ops[i];
+            std::fprintf(f, " %u/%zu/%d", (unsigned)o.kind, o.q.size(), __builtin_popcountll(o.nd));
+        }
+        std::fprintf(f, "\n");
+    }
+    std::fclose(f);
+}
+
 // Greedy pass builder.  An op joins the current pass when (a) its mixed qubits
 // fit in the tile together with those already chosen and (b) it commutes with
 // every op deferred so far in this scan (they either touch disjoint qubits or
@@ -111,7 +129,69 @@ std::vector<Stage> schedule(const std::vector<HOp>& ops, unsigned nq, SchedParam
         stages.push_back(st);
         remaining.swap(deferred);
     }
+    if (const char* path = std::getenv("FSA_SCHEDULE_DUMP")) dump_schedule(path, ops, stages);
     return stages;
+}
+
+// ---- swap relabeling ------------------------------------------------------------
+
+static uint64_t map_mask(uint64_t m, const unsigned* pos) {
+    uint64_t r = 0;
+    for (; m; m &= m - 1) r |= 1ULL << pos[__builtin_ctzll(m)];
+    return r;
+}
+
+// Logical amplitude i lives at the physical index whose bit pos[q] is bit q
+// of i.  A swap of logical qubits a, b only exchanges pos[a] and pos[b]; every
+// other op is rewritten onto physical qubits.  At the end, physical swaps move
+// each logical qubit home (at most nq - 1, chained through one position so
+// they pack densely into tile passes).  Swaps move data exactly, so results
+// are identical to executing the swaps in place.
+std::vector<HOp> relabel_swaps(const std::vector<HOp>& ops, unsigned nq) {
+    unsigned pos[64], who[64];
+    for (unsigned q = 0; q < 64; q++) { pos[q] = q; who[q] = q; }
+    std::vector<HOp> out;
+    out.reserve(ops.size() + nq);
+    for (const HOp& op : ops) {
+        if (op.kind == OP_SWAP) {
+            unsigned a = op.q[0], b = op.q[1];
+            std::swap(pos[a], pos[b]);
+            who[pos[a]] = a;
+            who[pos[b]] = b;
+            continue;
+        }
+        HOp o = op;
+        for (unsigned& q : o.q) q = pos[q];
+        o.ctrl = map_mask(o.ctrl, pos);
+        o.mxy = map_mask(o.mxy, pos);
+        o.myz = map_mask(o.myz, pos);
+        o.pmask = map_mask(o.pmask, pos);
+        finish_op(o);
+        out.push_back(std::move(o));
+    }
+    for (unsigned p = 0; p < nq; p++) {
+        while (who[p] != p) {
+            unsigned l = who[p];           // logical l sits at physical p; its home is physical l
+            unsigned lq = who[l];
+            HOp sw;
+            sw.kind = OP_SWAP;
+            sw.q = {std::min(p, l), std::max(p, l)};
+            finish_op(sw);
+            out.push_back(std::move(sw));
+            who[l] = l;
+            pos[l] = l;
+            who[p] = lq;
+            pos[lq] = p;
+        }
+    }
+    return out;
+}
+
+Plan make_plan(const std::vector<HOp>& ops, unsigned nq, int precision) {
+    Plan p;
+    p.ops = relabel_swaps(ops, nq);
+    p.stages = schedule(p.ops, nq, sched_params(nq, precision));
+    return p;
 }
 
 // ---- parameter ring ----------------------------------------------------------
@@ -368,11 +448,133 @@ void run_standalone(State& s, const HOp& op) {
     hip_check(hipGetLastError(), "kernel launch");
 }
 
+// ---- sweeps: runs of ops applied to register groups inside a tile pass -----
+
+bool sweepable(const HOp& op, unsigned W) {
+    switch (op.kind) {
+    case OP_MAT1: case OP_PHASE: case OP_SWAP: case OP_DEPHASE1: case OP_DEPHASE2: case OP_DEPOL1: case OP_DAMP:
+        return true;
+    case OP_PAULI: return (unsigned)__builtin_popcountll(op.mxy) <= W;
+    // dense gates in registers keep inputs and outputs live: 2 targets for FP64
+    // (W = 3), 3 for FP32 (W = 4); larger ones run as LDS items
+    case OP_MATK: return op.q.size() >= 2 && op.q.size() <= W - 1;
+    default: return false;
+    }
+}
+
+uint32_t tile_local(uint64_t gm, const int* pos) {
+    uint32_t lm = 0;
+    for (; gm; gm &= gm - 1) {
+        int p = pos[__builtin_ctzll(gm)];
+        if (p >= 0) lm |= 1u << p;
+    }
+    return lm;
+}
+
+struct MaskSplit {
+    uint32_t sweep = 0;   // sweep-local bits
+    uint32_t tile = 0;    // tile-local bits outside the sweep
+    uint64_t out = 0;     // global bits outside the tile
+};
+
+MaskSplit split_mask(uint64_t gm, const int* pos, const int* sl) {
+    MaskSplit r;
+    for (; gm; gm &= gm - 1) {
+        unsigned q = (unsigned)__builtin_ctzll(gm);
+        int lp = pos[q];
+        if (lp < 0) r.out |= 1ULL << q;
+        else if (sl[lp] >= 0) r.sweep |= 1u << sl[lp];
+        else r.tile |= 1u << lp;
+    }
+    return r;
+}
+
+template <typename R>
+TileOp translate_micro(const HOp& op, const int* pos, const int* sl, std::vector<cx<R>>& mats) {
+    TileOp t;
+    std::memset(&t, 0, sizeof(t));
+    t.kind = op.kind;
+    t.param = op.param < 0 ? NO_PARAM : (uint32_t)op.param;
+    auto sw = [&](unsigned q) { return (uint32_t)sl[pos[q]]; };
+    switch (op.kind) {
+    case OP_MAT1: {
+        t.p[0] = sw(op.q[0]);
+        MaskSplit c = split_mask(op.ctrl, pos, sl);
+        t.lmask = c.sweep; t.gq = c.tile; t.gmask = c.out;
+        for (int i = 0; i < 4; i++) { t.f[2 * i] = op.mat[i].real(); t.f[2 * i + 1] = op.mat[i].imag(); }
+        break;
+    }
+    case OP_PAULI: {
+        t.lmask = split_mask(op.mxy, pos, sl).sweep;
+        MaskSplit z = split_mask(op.myz, pos, sl);
+        t.lmask2 = z.sweep; t.gq = z.tile; t.gmask = z.out;
+        t.f[0] = op.th.real(); t.f[1] = op.th.imag();
+        t.f[2] = op.fe.real(); t.f[3] = op.fe.imag();
+        break;
+    }
+    case OP_PHASE: {
+        MaskSplit z = split_mask(op.pmask, pos, sl);
+        t.lmask = z.sweep; t.gq = z.tile; t.gmask = z.out;
+        t.f[0] = std::cos(op.theta); t.f[1] = std::sin(op.theta);
+        t.f[2] = std::cos(op.theta); t.f[3] = -std::sin(op.theta);
+        break;
+    }
+    case OP_SWAP: {
+        uint32_t a = sw(op.q[0]), b = sw(op.q[1]);
+        t.p[0] = std::min(a, b); t.p[1] = std::max(a, b);
+        break;
+    }
+    case OP_MATK: {
+        // matrix bit i acts on sweep bit s_i; reorder so bit j acts on the j-th smallest
+        const unsigned k = (unsigned)op.q.size();
+        std::vector<unsigned> si(k), order(k);
+        for (unsigned i = 0; i < k; i++) { si[i] = sw(op.q[i]); order[i] = i; }
+        std::sort(order.begin(), order.end(), [&](unsigned x, unsigned y) { return si[x] < si[y]; });
+        const uint64_t D = 1ULL << k;
+        auto orig = [&](uint64_t rNew) {
+            uint64_t r = 0;
+            for (unsigned j = 0; j < k; j++) if ((rNew >> j) & 1) r |= 1ULL << order[j];
+            return r;
+        };
+        t.k = k;
+        for (unsigned j = 0; j < k; j++) t.p[j] = si[order[j]];
+        t.mat_off = (uint32_t)mats.size();
+        for (uint64_t r = 0; r < D; r++)
+            for (uint64_t c = 0; c < D; c++) {
+                cd v = op.mat[orig(r) * D + orig(c)];
+                mats.push_back(mk<R>((R)v.real(), (R)v.imag()));
+            }
+        break;
+    }
+    case OP_DEPHASE1:
+    case OP_DEPHASE2:
+        for (size_t i = 0; i < op.q.size(); i++) {
+            unsigned q = op.q[i];
+            int lp = pos[q];
+            t.p[i] = lp < 0 ? q : sl[lp] >= 0 ? (0x200u | (uint32_t)sl[lp]) : (0x100u | (uint32_t)lp);
+        }
+        t.f[0] = op.kind == OP_DEPHASE1 ? op.c[0] : 1.0 + op.c[0];
+        break;
+    case OP_DEPOL1:
+    case OP_DAMP: {
+        uint32_t a = sw(op.q[0]), b = sw(op.q[1]);
+        t.p[0] = std::min(a, b); t.p[1] = std::max(a, b);
+        t.f[0] = op.c[0]; t.f[1] = op.c[1]; t.f[2] = op.c[2];
+        break;
+    }
+    default: break;
+    }
+    return t;
+}
+
 template <typename R>
 void execute_impl(State& s, const std::vector<HOp>& ops, const std::vector<Stage>& stages) {
-    std::vector<TileOp> tops;
+    constexpr unsigned W = (unsigned)sweep_bits<R>();
+    constexpr unsigned FULL = (unsigned)tile_bits_max<R>();
+    std::vector<TileOp> items, micro;
     std::vector<cx<R>> mats;
-    std::vector<size_t> first(stages.size(), 0);
+    std::vector<size_t> first(stages.size(), 0), count(stages.size(), 0);
+    std::vector<uint32_t> flags(stages.size(), 0);
     std::vector<TileGeom> geoms(stages.size());
     for (size_t si = 0; si < stages.size(); si++) {
         const Stage& st = stages[si];
@@ -381,34 +583,87 @@ void execute_impl(State& s, const std::vector<HOp>& ops, const std::vector<Stage
         for (int& p : pos) p = -1;
         TileGeom g;
         std::memset(&g, 0, sizeof(g));
-        g.m = (uint8_t)st.tq.size();
+        const unsigned m = (unsigned)st.tq.size();
+        g.m = (uint8_t)m;
         for (size_t i = 0; i < st.tq.size(); i++) {
             pos[st.tq[i]] = (int)i;
             g.q[i] = (uint8_t)st.tq[i];
             g.tileMask |= 1ULL << st.tq[i];
         }
         geoms[si] = g;
-        first[si] = tops.size();
-        for (int oi : st.ops) tops.push_back(translate<R>(ops[oi], pos, g.tileMask, mats));
+        first[si] = items.size();
+        // group consecutive sweepable ops whose mixed tile bits fit in W bits
+        struct Group { bool sweep; uint32_t bits; std::vector<int> ops; };
+        std::vector<Group> groups;
+        for (int oi : st.ops) {
+            const HOp& op = ops[oi];
+            if (m >= W && sweepable(op, W)) {
+                uint32_t nd = tile_local(op.nd, pos);
+                if (!groups.empty() && groups.back().sweep &&
+                    (unsigned)__builtin_popcount(groups.back().bits | nd) <= W) {
+                    groups.back().bits |= nd;
+                    groups.back().ops.push_back(oi);
+                } else {
+                    groups.push_back(Group{true, nd, {oi}});
+                }
+            } else {
+                groups.push_back(Group{false, 0, {oi}});
+            }
+        }
+        for (Group& gr : groups) {
+            if (!gr.sweep) {
+                items.push_back(translate<R>(ops[gr.ops[0]], pos, g.tileMask, mats));
+                continue;
+            }
+            // fill to W bits with the highest free tile bits (keeps the low, contiguous ones for lanes)
+            for (int b = (int)m - 1; b >= 0 && (unsigned)__builtin_popcount(gr.bits) < W; b--) gr.bits |= 1u << b;
+            int sl[32];
+            for (int& x : sl) x = -1;
+            TileOp it;
+            std::memset(&it, 0, sizeof(it));
+            it.kind = OP_SWEEP;
+            unsigned j = 0;
+            for (unsigned b = 0; b < m; b++)
+                if ((gr.bits >> b) & 1) { it.p[j] = b; sl[b] = (int)j; j++; }
+            it.k = (uint32_t)gr.ops.size();
+            it.mat_off = (uint32_t)micro.size();
+            it.param = NO_PARAM;
+            for (int oi : gr.ops) micro.push_back(translate_micro<R>(ops[oi], pos, sl, mats));
+            items.push_back(it);
+        }
+        count[si] = items.size() - first[si];
+        // fuse the first / last sweep with the HBM load / store on full tiles when
+        // its bits leave the 3 lowest tile bits to the lanes (8 consecutive amplitudes)
+        auto fusable = [&](const TileOp& it) { return it.kind == OP_SWEEP && m == FULL && it.p[0] >= 3; };
+        if (count[si] > 0) {
+            if (fusable(items[first[si]])) flags[si] |= 1u;
+            if (fusable(items[first[si] + count[si] - 1])) flags[si] |= 2u;
+            if (count[si] == 1 && flags[si] != 3u) flags[si] = 0;
+        }
     }
-    const TileOp* dtops = nullptr;
+    const TileOp* ditems = nullptr;
+    const TileOp* dmicro = nullptr;
     const cx<R>* dmats = nullptr;
-    if (!tops.empty()) {
-        size_t opBytes = tops.size() * sizeof(TileOp);
-        size_t matOff = (opBytes + 255) & ~(size_t)255;
+    if (!items.empty()) {
+        size_t itemBytes = items.size() * sizeof(TileOp);
+        size_t microOff = (itemBytes + 255) & ~(size_t)255;
+        size_t microBytes = micro.size() * sizeof(TileOp);
+        size_t matOff = (microOff + microBytes + 255) & ~(size_t)255;
         std::vector<char> blob(matOff + mats.size() * sizeof(cx<R>));
-        std::memcpy(blob.data(), tops.data(), opBytes);
+        std::memcpy(blob.data(), items.data(), itemBytes);
+        if (!micro.empty()) std::memcpy(blob.data() + microOff, micro.data(), microBytes);
         if (!mats.empty()) std::memcpy(blob.data() + matOff, mats.data(), mats.size() * sizeof(cx<R>));
         char* d = static_cast<char*>(s.ring.push(blob.data(), blob.size()));
-        dtops = reinterpret_cast<const TileOp*>(d);
+        ditems = reinterpret_cast<const TileOp*>(d);
+        dmicro = reinterpret_cast<const TileOp*>(d + microOff);
         dmats = reinterpret_cast<const cx<R>*>(d + matOff);
     }
     for (size_t si = 0; si < stages.size(); si++) {
         const Stage& st = stages[si];
         if (st.tiled) {
             ProfScope prof("tile_pass", s.stream);
-            launch_tile<R>(s.stream, static_cast<cx<R>*>(s.amps), s.nq, geoms[si], dtops + first[si],
-                           (unsigned)st.ops.size(), dmats);
+            launch_tile<R>(s.stream, static_cast<cx<R>*>(s.amps), s.nq, geoms[si], ditems + first[si],
+                           (unsigned)count[si], dmicro, dmats, flags[si]);
             hip_check(hipGetLastError(), "tile kernel launch");
         } else {
             for (int oi : st.ops) run_standalone<R>(s, ops[oi]);
@@ -438,7 +693,7 @@ void flush(State& s) {
     if (s.pending.empty()) return;
     std::vector<HOp> ops;
     ops.swap(s.pending);
-    execute(s, ops, schedule(ops, s.nq, sched_params(s.nq, s.prec)));
+    execute(s, make_plan(ops, s.nq, s.prec));
 }
 
 }  // namespace fsa

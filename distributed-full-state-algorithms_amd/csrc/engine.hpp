@@ -62,6 +62,15 @@ struct SchedParams {
 SchedParams sched_params(unsigned nq, int precision);
 std::vector<Stage> schedule(const std::vector<HOp>& ops, unsigned nq, SchedParams sp);
 
+// A fused plan: swap gates become relabelings of the later ops' qubits, the
+// accumulated qubit permutation is undone by swaps at the end, then passes.
+struct Plan {
+    std::vector<HOp> ops;
+    std::vector<Stage> stages;
+};
+std::vector<HOp> relabel_swaps(const std::vector<HOp>& ops, unsigned nq);
+Plan make_plan(const std::vector<HOp>& ops, unsigned nq, int precision);
+
 // pinned-host / device ring for per-launch parameters (matrices, tile ops)
 struct ParamRing {
     hipStream_t stream = nullptr;
@@ -95,14 +104,15 @@ struct Circuit {
     std::vector<HOp> ops;
     unsigned numParams = 0;
     unsigned maxQubit = 0;   // highest qubit index used + 1
-    // cached schedule per (nq, precision)
+    // cached plan per (nq, precision)
     unsigned cachedNq = 0;
     int cachedPrec = -1;
-    std::vector<Stage> cached;
+    Plan cached;
 };
 
 // run ops on the state's stream; eager: each op as its own pass(es)
 void execute(State& s, const std::vector<HOp>& ops, const std::vector<Stage>& stages);
+inline void execute(State& s, const Plan& p) { execute(s, p.ops, p.stages); }
 void execute_now(State& s, const std::vector<HOp>& ops);
 void flush(State& s);
 

@@ -51,18 +51,21 @@ __global__ __launch_bounds__(BLOCK) void k_swap(cx<R>* __restrict__ a, uint64_t 
                                                 uint64_t b2) {
     const uint64_t first = (uint64_t)blockIdx.x * (BLOCK * ITEMS) + threadIdx.x;
     const uint64_t m = b1 | b2;
-    cx<R> va[ITEMS], vb[ITEMS];
+    R ax[ITEMS], ay[ITEMS], bx[ITEMS], by[ITEMS];    // split re/im keeps them in registers
     uint64_t idx[ITEMS];
 #pragma unroll
     for (int p = 0; p < ITEMS; p++) {
         uint64_t k = first + (uint64_t)p * BLOCK;
         idx[p] = ins0_mask(k, m);
-        if (k < numIts) { va[p] = a[idx[p] | b1]; vb[p] = a[idx[p] | b2]; }
+        if (k < numIts) {
+            cx<R> u = a[idx[p] | b1], v = a[idx[p] | b2];
+            ax[p] = u.x; ay[p] = u.y; bx[p] = v.x; by[p] = v.y;
+        }
     }
 #pragma unroll
     for (int p = 0; p < ITEMS; p++) {
         uint64_t k = first + (uint64_t)p * BLOCK;
-        if (k < numIts) { a[idx[p] | b1] = vb[p]; a[idx[p] | b2] = va[p]; }
+        if (k < numIts) { a[idx[p] | b1] = mk<R>(bx[p], by[p]); a[idx[p] | b2] = mk<R>(ax[p], ay[p]); }
     }
 }
 

@@ -37,11 +37,15 @@ template <typename R>
 void launch_scale(hipStream_t s, cx<R>* a, uint64_t num, R f);
 
 // ---- fused tile pass ----------------------------------------------------------
+// items: plain ops or OP_SWEEP items whose micro-ops live in dmicro; flags bit 0:
+// the first item is a sweep fused with the HBM load, bit 1: the last with the store
 template <typename R>
-void launch_tile(hipStream_t s, cx<R>* a, unsigned nq, const TileGeom& geom, const TileOp* dops,
-                 unsigned numOps, const cx<R>* dmats);
+void launch_tile(hipStream_t s, cx<R>* a, unsigned nq, const TileGeom& geom, const TileOp* ditems,
+                 unsigned numItems, const TileOp* dmicro, const cx<R>* dmats, uint32_t flags);
 // largest tile (log2 amplitudes) the LDS budget allows for this precision
 template <typename R> constexpr int tile_bits_max() { return sizeof(R) == 8 ? 12 : 13; }
+// register-sweep width: 2^W amplitudes per thread = full tile / 512 threads
+template <typename R> constexpr int sweep_bits() { return sizeof(R) == 8 ? 3 : 4; }
 
 // ---- init / copy / conversion ------------------------------------------------
 template <typename R>
