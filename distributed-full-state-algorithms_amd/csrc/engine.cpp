@@ -57,6 +57,16 @@ static unsigned env_u(const char* name, unsigned dflt) {
     return (unsigned)std::strtoul(v, nullptr, 10);
 }
 
+// Tuning switches (diagnostics): FSA_MFMA=0 runs dense k=3..5 gates on the
+// VALU form; FSA_SWEEPS=1 enables register sweeps for passes without dense
+// gates.  Measured on a 30-qubit FP64 circuit of 1000 gates the defaults
+// (MFMA on, sweeps off) are the fastest combination.
+static bool env_flag(const char* name, bool dflt) {
+    const char* v = std::getenv(name);
+    if (!v || !*v) return dflt;
+    return v[0] != '0';
+}
+
 SchedParams sched_params(unsigned nq, int precision) {
     unsigned mmax = precision == FSA_FP64 ? (unsigned)tile_bits_max<double>() : (unsigned)tile_bits_max<float>();
     unsigned m = std::min(env_u("FSA_TILE_BITS", mmax), mmax);
@@ -346,12 +356,34 @@ TileOp translate(const HOp& op, const int* pos, uint64_t tileMask, std::vector<c
         t.gmask = op.ctrl & ~tileMask;
         for (int i = 0; i < 4; i++) { t.f[2 * i] = op.mat[i].real(); t.f[2 * i + 1] = op.mat[i].imag(); }
         break;
-    case OP_MATK:
-        t.k = (uint32_t)op.q.size();
+    case OP_MATK: {
+        const unsigned k = (unsigned)op.q.size();
+        t.k = k;
         for (size_t i = 0; i < op.q.size(); i++) t.p[i] = (uint32_t)pos[op.q[i]];
         t.mat_off = (uint32_t)mats.size();
-        for (const cd& v : op.mat) mats.push_back(to_cx<R>(v));
+        unsigned m = 0;
+        for (int i = 0; i < 64; i++) if (pos[i] >= 0) m++;
+        static const bool useMfma = env_flag("FSA_MFMA", true);
+        if (useMfma && k >= 3 && k <= 5 && m == (unsigned)tile_bits_max<R>()) {
+            // MFMA form: real 2D x 2D matrix [[Ur, -Ui], [Ui, Ur]], row-major, in
+            // complex slots of the parameter buffer (two reals per slot)
+            t.lmask2 = 1;
+            const uint64_t D = 1ULL << k, D2 = 2 * D;
+            std::vector<R> real(D2 * D2);
+            for (uint64_t r = 0; r < D; r++)
+                for (uint64_t c = 0; c < D; c++) {
+                    cd u = op.mat[r * D + c];
+                    real[r * D2 + c] = (R)u.real();
+                    real[r * D2 + D + c] = (R)-u.imag();
+                    real[(D + r) * D2 + c] = (R)u.imag();
+                    real[(D + r) * D2 + D + c] = (R)u.real();
+                }
+            for (uint64_t i = 0; i < real.size(); i += 2) mats.push_back(mk<R>(real[i], real[i + 1]));
+        } else {
+            for (const cd& v : op.mat) mats.push_back(to_cx<R>(v));
+        }
         break;
+    }
     case OP_SWAP:
         t.p[0] = (uint32_t)pos[op.q[0]];
         t.p[1] = (uint32_t)pos[op.q[1]];
@@ -595,9 +627,17 @@ void execute_impl(State& s, const std::vector<HOp>& ops, const std::vector<Stage
         // group consecutive sweepable ops whose mixed tile bits fit in W bits
         struct Group { bool sweep; uint32_t bits; std::vector<int> ops; };
         std::vector<Group> groups;
+        static const bool useSweeps = env_flag("FSA_SWEEPS", false);
+        // passes with dense gates (k >= 3) or two-qubit depolarising run as
+        // plain LDS items on the dense kernel variant; sweeps serve the others
+        bool denseStage = false;
         for (int oi : st.ops) {
             const HOp& op = ops[oi];
-            if (m >= W && sweepable(op, W)) {
+            if ((op.kind == OP_MATK && !sweepable(op, W)) || op.kind == OP_DEPOL2) denseStage = true;
+        }
+        for (int oi : st.ops) {
+            const HOp& op = ops[oi];
+            if (useSweeps && !denseStage && m >= W && sweepable(op, W)) {
                 uint32_t nd = tile_local(op.nd, pos);
                 if (!groups.empty() && groups.back().sweep &&
                     (unsigned)__builtin_popcount(groups.back().bits | nd) <= W) {
@@ -640,6 +680,16 @@ void execute_impl(State& s, const std::vector<HOp>& ops, const std::vector<Stage
             if (fusable(items[first[si] + count[si] - 1])) flags[si] |= 2u;
             if (count[si] == 1 && flags[si] != 3u) flags[si] = 0;
         }
+        // kernel variant: 1 for dense gates / two-qubit depolarising; bit 4 marks sweeps
+        uint32_t var = 0;
+        bool hasSweep = false;
+        for (size_t i = first[si]; i < first[si] + count[si]; i++) {
+            if (items[i].kind == OP_MATK) var = std::max(var, items[i].lmask2 == 1u ? 2u : 1u);
+            if (items[i].kind == OP_DEPOL2) var = std::max(var, 1u);
+            if (items[i].kind == OP_SWEEP) hasSweep = true;
+        }
+        flags[si] |= var << 2;
+        if (hasSweep) flags[si] |= 16u;
     }
     const TileOp* ditems = nullptr;
     const TileOp* dmicro = nullptr;

@@ -120,7 +120,100 @@ __device__ __forceinline__ void tile_matk(cx<R>* s, uint32_t T, const TileOp& op
 #undef FSA_OFF
 }
 
-template <typename R, int TB>
+// ---- dense gates on MFMA ----------------------------------------------------
+// Y = U X for the 2^K amplitudes of every group in the tile, as the real GEMM
+// [Yr; Yi] = [[Ur, -Ui], [Ui, Ur]] [Xr; Xi] (2D x 2D times 2D x groups, D = 2^K)
+// on 16x16x4 MFMA tiles.  Operand layout of v_mfma_*_16x16x4: lane l holds
+// A[l % 16][l / 16] and B[l / 16][l % 16]; accumulator element j of lane l is
+// column l % 16, row 4 j + l / 16 for f64 and row 4 (l / 16) + j for f32
+// (measured on gfx950).  A wave task is one 16-row block
+// of the real matrix against four 16-group column blocks; the real matrix (in
+// the parameter buffer, row-major 2D x 2D) feeds A, the tile in LDS feeds B.
+// All inputs are read before the barrier, outputs written after it.
+typedef double fsa_d4 __attribute__((ext_vector_type(4)));
+typedef float fsa_f4 __attribute__((ext_vector_type(4)));
+template <typename R> struct MfmaAcc;
+template <> struct MfmaAcc<double> {
+    typedef fsa_d4 T;
+    static __device__ __forceinline__ T step(double a, double b, T c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+};
+template <> struct MfmaAcc<float> {
+    typedef fsa_f4 T;
+    static __device__ __forceinline__ T step(float a, float b, T c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+};
+
+// tasks per wave for a full tile: FP64 (4096 amplitudes) 1, FP32 (8192) 2
+template <typename R, int K, int TB>
+__device__ __forceinline__ void tile_matk_mfma(cx<R>* s, uint32_t T, const TileOp& op, const R* __restrict__ M) {
+    typedef typename MfmaAcc<R>::T acc_t;
+    constexpr int D = 1 << K, D2 = 2 * D;
+    constexpr int RB = D2 / 16;                 // 16-row blocks of the real matrix
+    constexpr int WAVES = TB / 64;
+    constexpr int MAXT = sizeof(R) == 8 ? 1 : 2;
+    const uint32_t b0 = 1u << op.p[0], b1 = 1u << op.p[1], b2 = 1u << op.p[2];
+    const uint32_t b3 = K > 3 ? 1u << op.p[3] : 0u;
+    const uint32_t b4 = K > 4 ? 1u << op.p[4] : 0u;
+    const uint32_t tm = b0 | b1 | b2 | b3 | b4;
+    const uint32_t groups = T >> K;
+    const uint32_t cb4 = groups / 64;          // groups of four 16-column blocks
+    const uint32_t tasks = RB * cb4;
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    const uint32_t lr = lane & 15u, lk = lane >> 4;
+    acc_t acc[MAXT][4];
+    uint32_t taskId[MAXT];
+#pragma unroll
+    for (int t = 0; t < MAXT; t++) {
+        taskId[t] = wave + (uint32_t)t * WAVES;
+#pragma unroll
+        for (int c = 0; c < 4; c++) acc[t][c] = acc_t{0, 0, 0, 0};
+        if (taskId[t] >= tasks) continue;
+        const uint32_t rb = taskId[t] % RB, cbase = (taskId[t] / RB) * 4;
+        const R* arow = M + (size_t)(rb * 16 + lr) * D2;
+        // B-fragment group bases of the four column blocks for this lane
+        uint32_t gb[4];
+#pragma unroll
+        for (int c = 0; c < 4; c++) gb[c] = ins0_mask32((cbase + (uint32_t)c) * 16 + lr, tm);
+        const R* sr = reinterpret_cast<const R*>(s);
+        // four k-steps per iteration: their A loads and LDS reads issue together
+#pragma unroll 4
+        for (int ks = 0; ks < D2 / 4; ks++) {
+            const R a = arow[ks * 4 + lk];
+            const uint32_t r = (uint32_t)ks * 4 + lk;          // real row of X this lane feeds
+            const uint32_t off = mat_offset<K>(r & (D - 1), b0, b1, b2, b3, b4);
+            const uint32_t half = r < (uint32_t)D ? 0u : 1u;  // real or imaginary part only
+#pragma unroll
+            for (int c = 0; c < 4; c++) acc[t][c] = MfmaAcc<R>::step(a, sr[2 * (gb[c] | off) + half], acc[t][c]);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int t = 0; t < MAXT; t++) {
+        if (taskId[t] >= tasks) continue;
+        const uint32_t rb = taskId[t] % RB, cbase = (taskId[t] / RB) * 4;
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            const uint32_t gbc = ins0_mask32((cbase + (uint32_t)c) * 16 + lr, tm);
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                // accumulator row j of this lane: FP64 4j + lane/16, FP32 4 (lane/16) + j
+                const uint32_t row = rb * 16 + (sizeof(R) == 8 ? 4u * (uint32_t)j + lk : lk * 4u + (uint32_t)j);
+                const uint32_t l = row & (D - 1);
+                R* dst = reinterpret_cast<R*>(&s[gbc | mat_offset<K>(l, b0, b1, b2, b3, b4)]);
+                dst[row < (uint32_t)D ? 0 : 1] = acc[t][c][j];
+            }
+        }
+    }
+}
+
+// Kernel variants: VAR 0 carries the simple LDS ops (and, with SW, register
+// sweeps); VAR 1 adds VALU dense gates and two-qubit depolarising; VAR 2 has
+// MFMA dense gates instead of VALU ones.  Each instantiation stays within 128
+// VGPRs so two 64 KiB workgroups share a CU.
+template <typename R, int TB, int VAR>
 __device__ __forceinline__ void tile_apply(cx<R>* s, uint32_t T, uint64_t base, const TileOp& op,
                            const cx<R>* __restrict__ mats) {
     const uint32_t tid = threadIdx.x;
@@ -181,7 +274,18 @@ __device__ __forceinline__ void tile_apply(cx<R>* s, uint32_t T, uint64_t base, 
         return;
     }
     case OP_MATK: {
+        if constexpr (VAR == 0) return;
         const cx<R>* mat = mats + op.mat_off;
+        if (op.lmask2 == 1u) {        // host chose the MFMA form (full tile, k = 3, 4 or 5)
+            if constexpr (VAR == 2) {
+                const R* M = reinterpret_cast<const R*>(mat);
+                if (op.k == 3) tile_matk_mfma<R, 3, TB>(s, T, op, M);
+                else if (op.k == 4) tile_matk_mfma<R, 4, TB>(s, T, op, M);
+                else tile_matk_mfma<R, 5, TB>(s, T, op, M);
+            }
+            return;
+        }
+        if constexpr (VAR == 2) return;
         switch (op.k) {
         case 1: tile_matk<R, 1, TB>(s, T, op, mat); break;
         case 2: tile_matk<R, 2, TB>(s, T, op, mat); break;
@@ -235,6 +339,7 @@ __device__ __forceinline__ void tile_apply(cx<R>* s, uint32_t T, uint64_t base, 
         return;
     }
     case OP_DEPOL2: {
+        if constexpr (VAR == 0) return;
         const R c1 = (R)op.f[0], c2 = (R)op.f[1], sc = (R)op.f[2];
         const uint32_t b0 = 1u << op.p[0], b1 = 1u << op.p[1], b2 = 1u << op.p[2], b3 = 1u << op.p[3];
         const uint32_t mk4 = b0 | b1 | b2 | b3;
@@ -546,7 +651,7 @@ __device__ __forceinline__ uint64_t tile_scatter(uint32_t l, const TileGeom& g) 
 // sweep fused with the HBM load (registers in that sweep's layout); FOUT: the
 // last item is a sweep fused with the HBM store.  A pass made of a single
 // fused sweep never touches LDS.  ROWS = tile size / TB for row-layout loads.
-template <typename R, int TB, int ROWS, int W>
+template <typename R, int TB, int ROWS, int W, int VAR, bool SW>
 __global__ __launch_bounds__(TB, 4) void k_tile(cx<R>* __restrict__ a, TileGeom geom,
                                                 const TileOp* __restrict__ items, unsigned numItems,
                                                 const TileOp* __restrict__ micro, const cx<R>* __restrict__ mats,
@@ -561,7 +666,7 @@ __global__ __launch_bounds__(TB, 4) void k_tile(cx<R>* __restrict__ a, TileGeom 
     const uint32_t T = 1u << m;
     const uint32_t tid = threadIdx.x;
     const bool owner = tid < T;
-    const bool fin = (flags & 1u) != 0, fout = (flags & 2u) != 0;
+    const bool fin = SW && (flags & 1u) != 0, fout = SW && (flags & 2u) != 0;
 
     // Addresses are ORs of per-bit values selected by compile-time register
     // indices, so only the per-bit values stay live.
@@ -620,8 +725,8 @@ __global__ __launch_bounds__(TB, 4) void k_tile(cx<R>* __restrict__ a, TileGeom 
         }
         for (unsigned it = 0; it < numItems; it++) {
             const TileOp& op = items[it];
-            if (op.kind != OP_SWEEP) {
-                tile_apply<R, TB>(s, T, base, op, mats);
+            if (!SW || op.kind != OP_SWEEP) {
+                tile_apply<R, TB, VAR>(s, T, base, op, mats);
                 __syncthreads();
                 continue;
             }
@@ -690,14 +795,20 @@ void launch_tile(hipStream_t s, cx<R>* a, unsigned nq, const TileGeom& geom, con
     // enough resident workgroups to fill every CU (LDS allows two 64 KiB tiles per CU)
     uint64_t grid = std::min<uint64_t>(numTiles, (uint64_t)numCU * 2);
     const uint32_t rows = geom.m > 9 ? 1u << (geom.m - 9) : 1u;
-    auto* kern = k_tile<R, TB, 1, W>;
+    const uint32_t var = (flags >> 2) & 3u;
+    const bool sw = (flags & 16u) != 0;
+    auto* kern = k_tile<R, TB, 1, W, 0, false>;
+#define FSA_PICK(RW)                                                                              \
+    kern = var == 2 ? k_tile<R, TB, RW, W, 2, false> : var == 1 ? k_tile<R, TB, RW, W, 1, false>          \
+         : sw ? k_tile<R, TB, RW, W, 0, true> : k_tile<R, TB, RW, W, 0, false>;
     switch (rows) {
-    case 1: kern = k_tile<R, TB, 1, W>; break;
-    case 2: kern = k_tile<R, TB, 2, W>; break;
-    case 4: kern = k_tile<R, TB, 4, W>; break;
-    case 8: kern = k_tile<R, TB, 8, W>; break;
-    default: kern = k_tile<R, TB, (sizeof(R) == 8 ? 8 : 16), W>; break;
+    case 1: FSA_PICK(1) break;
+    case 2: FSA_PICK(2) break;
+    case 4: FSA_PICK(4) break;
+    case 8: FSA_PICK(8) break;
+    default: FSA_PICK((sizeof(R) == 8 ? 8 : 16)) break;
     }
+#undef FSA_PICK
     hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(TB), lds, s, a, geom, ditems, numItems, dmicro, dmats,
                        numTiles, flags);
 }

@@ -38,7 +38,9 @@ void launch_scale(hipStream_t s, cx<R>* a, uint64_t num, R f);
 
 // ---- fused tile pass ----------------------------------------------------------
 // items: plain ops or OP_SWEEP items whose micro-ops live in dmicro; flags bit 0:
-// the first item is a sweep fused with the HBM load, bit 1: the last with the store
+// the first item is a sweep fused with the HBM load, bit 1: the last with the
+// store, bits 2-3: kernel variant (0 simple ops, 1 adds VALU dense gates and
+// two-qubit depolarising, 2 MFMA dense gates), bit 4: the pass has sweeps
 template <typename R>
 void launch_tile(hipStream_t s, cx<R>* a, unsigned nq, const TileGeom& geom, const TileOp* ditems,
                  unsigned numItems, const TileOp* dmicro, const cx<R>* dmats, uint32_t flags);
