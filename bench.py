@@ -98,6 +98,29 @@ def cpu_baseline(circ, n, ops, seed, budget_s=15.0):
                       f"({dt:.1f} s, {nbytes / dt / 1e9:.1f} GB/s algorithmic)"}
 
 
+def copy_calibration(n, reps=5):
+    """A plain device-to-device copy of one n-qubit FP64 state (torch copy_),
+    timed with events: the practical streaming ceiling next to the 8 TB/s peak,
+    and the known-size kernel that scales the FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+
+    nbytes = 16 << n
+    src = torch.empty(nbytes // 8, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    src.normal_()
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0.record()
+    for _ in range(reps):
+        dst.copy_(src)
+    t1.record()
+    torch.cuda.synchronize()
+    ms = t0.elapsed_time(t1) / reps
+    print(json.dumps({"copy_bytes_read": nbytes, "copy_bytes_written": nbytes, "ms_per_copy": round(ms, 4),
+                      "GB_per_s": round(2 * nbytes / (ms * 1e-3) / 1e9, 1)}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -107,12 +130,18 @@ def main():
     ap.add_argument("--gates", type=int, default=1000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-per-type", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only time a plain copy of one state-sized buffer and exit")
     a = ap.parse_args()
 
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); the multi-GPU exchange "
               "layer is not part of this build", file=sys.stderr)
         sys.exit(2)
+
+    if a.calibrate_copy:
+        copy_calibration(a.qubits)
+        return
 
     fsa = ge.load_package()
     circ = load_circuits_module()

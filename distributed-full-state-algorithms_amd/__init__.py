@@ -26,7 +26,7 @@ FSA_EXEC_EAGER, FSA_EXEC_FUSED = 0, 1
 I, X, Y, Z = 0, 1, 2, 3          # reference src/types.hpp:16-18
 
 ERR_NAMES = {1: "FSA_ERR_ARG", 2: "FSA_ERR_QUBIT", 3: "FSA_ERR_DUPLICATE", 4: "FSA_ERR_SIZE",
-             5: "FSA_ERR_OOM", 6: "FSA_ERR_HIP", 7: "FSA_ERR_KIND"}
+             5: "FSA_ERR_OOM", 6: "FSA_ERR_HIP", 7: "FSA_ERR_KIND", 8: "FSA_ERR_INTERNAL"}
 
 
 class FsaError(RuntimeError):

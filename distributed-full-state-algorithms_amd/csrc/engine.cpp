@@ -339,7 +339,7 @@ uint32_t chan_code(unsigned q, const int* pos) {
 }
 
 template <typename R>
-TileOp translate(const HOp& op, const int* pos, uint64_t tileMask, std::vector<cx<R>>& mats) {
+TileOp translate(const HOp& op, const int* pos, uint64_t tileMask, std::vector<cx<R>>& mats, bool mfma) {
     TileOp t;
     std::memset(&t, 0, sizeof(t));
     t.kind = op.kind;
@@ -361,10 +361,7 @@ TileOp translate(const HOp& op, const int* pos, uint64_t tileMask, std::vector<c
         t.k = k;
         for (size_t i = 0; i < op.q.size(); i++) t.p[i] = (uint32_t)pos[op.q[i]];
         t.mat_off = (uint32_t)mats.size();
-        unsigned m = 0;
-        for (int i = 0; i < 64; i++) if (pos[i] >= 0) m++;
-        static const bool useMfma = env_flag("FSA_MFMA", true);
-        if (useMfma && k >= 3 && k <= 5 && m == (unsigned)tile_bits_max<R>()) {
+        if (mfma) {
             // MFMA form: real 2D x 2D matrix [[Ur, -Ui], [Ui, Ur]], row-major, in
             // complex slots of the parameter buffer (two reals per slot)
             t.lmask2 = 1;
@@ -627,7 +624,7 @@ void execute_impl(State& s, const std::vector<HOp>& ops, const std::vector<Stage
         // group consecutive sweepable ops whose mixed tile bits fit in W bits
         struct Group { bool sweep; uint32_t bits; std::vector<int> ops; };
         std::vector<Group> groups;
-        static const bool useSweeps = env_flag("FSA_SWEEPS", false);
+        const bool useSweeps = env_flag("FSA_SWEEPS", false);
         // passes with dense gates (k >= 3) or two-qubit depolarising run as
         // plain LDS items on the dense kernel variant; sweeps serve the others
         bool denseStage = false;
@@ -635,6 +632,18 @@ void execute_impl(State& s, const std::vector<HOp>& ops, const std::vector<Stage
             const HOp& op = ops[oi];
             if ((op.kind == OP_MATK && !sweepable(op, W)) || op.kind == OP_DEPOL2) denseStage = true;
         }
+        // the MFMA kernel variant carries no VALU dense gates: a pass uses it only
+        // when every dense gate in it has the MFMA form (k = 3..5 on a full tile)
+        const bool useMfma = env_flag("FSA_MFMA", true);
+        bool stageMfma = useMfma && m == FULL;
+        bool anyMfma = false;
+        for (int oi : st.ops) {
+            const HOp& op = ops[oi];
+            if (op.kind != OP_MATK) continue;
+            if (op.q.size() >= 3 && op.q.size() <= 5) anyMfma = true;
+            else stageMfma = false;
+        }
+        stageMfma = stageMfma && anyMfma;
         for (int oi : st.ops) {
             const HOp& op = ops[oi];
             if (useSweeps && !denseStage && m >= W && sweepable(op, W)) {
@@ -652,7 +661,7 @@ void execute_impl(State& s, const std::vector<HOp>& ops, const std::vector<Stage
         }
         for (Group& gr : groups) {
             if (!gr.sweep) {
-                items.push_back(translate<R>(ops[gr.ops[0]], pos, g.tileMask, mats));
+                items.push_back(translate<R>(ops[gr.ops[0]], pos, g.tileMask, mats, stageMfma));
                 continue;
             }
             // fill to W bits with the highest free tile bits (keeps the low, contiguous ones for lanes)
@@ -688,6 +697,9 @@ void execute_impl(State& s, const std::vector<HOp>& ops, const std::vector<Stage
             if (items[i].kind == OP_DEPOL2) var = std::max(var, 1u);
             if (items[i].kind == OP_SWEEP) hasSweep = true;
         }
+        for (size_t i = first[si]; i < first[si] + count[si]; i++)
+            if (items[i].kind == OP_MATK && (items[i].lmask2 == 1u) != (var == 2u))
+                fail(FSA_ERR_INTERNAL, "tile pass mixes MFMA and VALU dense gates");
         flags[si] |= var << 2;
         if (hasSweep) flags[si] |= 16u;
     }
@@ -711,7 +723,11 @@ void execute_impl(State& s, const std::vector<HOp>& ops, const std::vector<Stage
     for (size_t si = 0; si < stages.size(); si++) {
         const Stage& st = stages[si];
         if (st.tiled) {
-            ProfScope prof("tile_pass", s.stream);
+            // one profile name per kernel instantiation family, so the timings
+            // line up with per-kernel profiler statistics
+            const uint32_t var = (flags[si] >> 2) & 3u;
+            ProfScope prof(var == 2 ? "tile_pass_mfma" : var == 1 ? "tile_pass_dense"
+                           : (flags[si] & 16u) ? "tile_pass_sweep" : "tile_pass_plain", s.stream);
             launch_tile<R>(s.stream, static_cast<cx<R>*>(s.amps), s.nq, geoms[si], ditems + first[si],
                            (unsigned)count[si], dmicro, dmats, flags[si]);
             hip_check(hipGetLastError(), "tile kernel launch");

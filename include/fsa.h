@@ -42,7 +42,8 @@ enum {
     FSA_ERR_SIZE = 4,       /* too many qubits/targets for this build or device */
     FSA_ERR_OOM = 5,        /* device or pinned-host allocation failed */
     FSA_ERR_HIP = 6,        /* a HIP runtime call failed */
-    FSA_ERR_KIND = 7        /* statevector op on a density matrix or vice versa */
+    FSA_ERR_KIND = 7,       /* statevector op on a density matrix or vice versa */
+    FSA_ERR_INTERNAL = 8    /* an engine invariant failed (a bug in libfsa) */
 };
 
 enum { FSA_FP64 = 0, FSA_FP32 = 1 };

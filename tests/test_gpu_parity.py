@@ -83,9 +83,18 @@ def test_sv_single_ops(fsa, orc, mode, n):
         assert err <= TOL64, (n, t, name, args[:-1] if name != "swapGate" else args, err)
 
 
+TUNINGS = {"default": {}, "valu-dense": {"FSA_MFMA": "0"}, "sweeps": {"FSA_SWEEPS": "1"},
+           "sweeps-valu": {"FSA_SWEEPS": "1", "FSA_MFMA": "0"}}
+
+
 @pytest.mark.parametrize("n", [4, 9, 12, 14, 18, 20])
-def test_sv_fused_circuits(fsa, orc, n):
-    """Long random circuits in fused mode: scheduler + tile kernel, one readback."""
+@pytest.mark.parametrize("tuning", list(TUNINGS))
+def test_sv_fused_circuits(fsa, orc, n, tuning, monkeypatch):
+    """Long random circuits in fused mode: scheduler + tile kernel, one readback.
+    Every tuning switch of the pass builder (dense gates on MFMA or VALU,
+    register sweeps on or off) must give the same amplitudes."""
+    for k, v in TUNINGS[tuning].items():
+        monkeypatch.setenv(k, v)
     rng = np.random.default_rng(77 + n)
     psi = fsa.StateVector(n, mode=1)
     v = rs(rng, n)
@@ -246,7 +255,12 @@ def test_gaussian_init_matches_oracle_law(fsa, orc):
 
 
 @pytest.mark.parametrize("n", [6, 14, 19])
-def test_fp32_bulk_against_fp64_oracle(fsa, orc, n):
+@pytest.mark.parametrize("tuning", list(TUNINGS))
+def test_fp32_bulk_against_fp64_oracle(fsa, orc, n, tuning, monkeypatch):
+    """FP32 amplitudes (every kernel form, dense gates included) against the
+    FP64 oracle, with unitary gates so the error does not grow with the norm."""
+    for k, v in TUNINGS[tuning].items():
+        monkeypatch.setenv(k, v)
     rng = np.random.default_rng(600 + n)
     for mode in MODES:
         psi = fsa.StateVector(n, precision="fp32", mode=mode)
@@ -254,8 +268,8 @@ def test_fp32_bulk_against_fp64_oracle(fsa, orc, n):
         psi.upload(v)
         ref = v.copy()
         for _ in range(40):
-            name, args = random_sv_op(rng, n, kinds=("one", "ctrl", "swap", "gadget", "phase", "tensor"))
-            if name == "oneTargGate" or name == "manyCtrlOneTargGate":
+            name, args = random_sv_op(rng, n, kinds=("one", "ctrl", "swap", "gadget", "phase", "tensor", "many"))
+            if name in ("oneTargGate", "manyCtrlOneTargGate", "manyTargGate"):
                 g = args[-1]
                 u, _, vh = np.linalg.svd(g)
                 args = args[:-1] + (u @ vh,)
