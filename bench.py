@@ -98,6 +98,42 @@ def cpu_baseline(circ, n, ops, seed, budget_s=15.0):
                       f"({dt:.1f} s, {nbytes / dt / 1e9:.1f} GB/s algorithmic)"}
 
 
+def adjoint_bench(fsa, circ, n=24, reps=3):
+    """Layered ZZ + X gadget circuit from |0..0> (20 layers, 940 angles at 24
+    qubits), H = 100 Pauli strings: <H> and the adjoint gradient of every angle,
+    FP64 and FP32-bulk amplitudes (FP64 sums).  circuits/s counts whole
+    forward + reverse sweeps; errors are FP32 against FP64 on the same GPU."""
+    ops, coeffs, paulis = circ.config4(n=n)
+    c = fsa.Circuit(ops)
+    out = {"workload": f"{n}-qubit |0..0>, 20 layers of ZZ + X gadgets ({c.num_params} angles), "
+                       f"H = {len(coeffs)} Pauli strings of weight 1-4"}
+    res = {}
+    for prec in ("fp64", "fp32"):
+        psi0 = fsa.StateVector(n, precision=prec)
+        psi0.init_zero()
+        fsa.adjoint_gradient(c, psi0, coeffs, paulis)        # warm-up
+        fsa.profile_enable(True)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            e, g = fsa.adjoint_gradient(c, psi0, coeffs, paulis)
+        dt = (time.perf_counter() - t0) / reps
+        prof = fsa.profile_read()
+        fsa.profile_enable(False)
+        psi0.close()
+        res[prec] = (e, g)
+        out[prec] = {"circuits_per_s": round(1.0 / dt, 3), "ms_per_circuit": round(dt * 1e3, 2),
+                     "kernel_ms_per_circuit": {k: round(v[1] / reps, 2) for k, v in
+                                               sorted(prof.items(), key=lambda kv: -kv[1][1])[:6]}}
+        log(f"adjoint {prec}: {dt * 1e3:.1f} ms per circuit")
+    e64, g64 = res["fp64"]
+    e32, g32 = res["fp32"]
+    out["fp32_speedup"] = round(out["fp32"]["circuits_per_s"] / out["fp64"]["circuits_per_s"], 3)
+    out["fp32_grad_rel_err_vs_fp64"] = float(np.linalg.norm(g32 - g64) / np.linalg.norm(g64))
+    out["fp32_expec_rel_err_vs_fp64"] = float(abs(e32 - e64) / abs(e64))
+    out["expec_fp64"] = e64
+    return out
+
+
 def copy_calibration(n, reps=5):
     """A plain device-to-device copy of one n-qubit FP64 state (torch copy_),
     timed with events: the practical streaming ceiling next to the 8 TB/s peak,
@@ -130,6 +166,7 @@ def main():
     ap.add_argument("--gates", type=int, default=1000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-per-type", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only time a plain copy of one state-sized buffer and exit")
     a = ap.parse_args()
@@ -230,6 +267,9 @@ def main():
         },
         "kernel_time_ms": {k: round(v[1], 2) for k, v in prof.items()},
     }
+    if not a.no_adjoint:
+        log("adjoint gradient, FP64 and FP32-bulk")
+        result["adjoint"] = adjoint_bench(fsa, circ)
     if not a.no_per_type:
         log("per-gate-type eager timing")
         result["per_gate_type"] = per_gate_type(fsa, circ, n, ops)
