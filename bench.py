@@ -134,6 +134,109 @@ def adjoint_bench(fsa, circ, n=24, reps=3):
     return out
 
 
+def dm_bench(fsa, circ, N=14, seed=20261205):
+    """14-qubit density matrix (2^28 complex128 = 4 GiB), rank-8 random mixed
+    state, 400 channel / unitary ops in fused passes, then expecPauliString
+    over 64 strings and partialTrace over 4 qubits."""
+    ops, coeffs, paulis, tt = circ.config3(seed=seed, N=N)
+    rho_h = circ.random_mixed_state(N, seed)
+    rho = fsa.DensityMatrix(N)
+    rho.upload(rho_h)
+    del rho_h
+    fsa.apply_ops(rho, ops, dm=True)                  # warm-up (kernels, schedule)
+    rho.sync()
+    fsa.profile_enable(True)
+    t0 = time.perf_counter()
+    fsa.apply_ops(rho, ops, dm=True)
+    rho.sync()
+    t_ops = time.perf_counter() - t0
+    prof = fsa.profile_read()
+    fsa.profile_enable(False)
+    fsa.distributed_densitymatrix_expecPauliString(rho, coeffs, paulis)     # warm-up
+    t0 = time.perf_counter()
+    reps = 5
+    for _ in range(reps):
+        ex = fsa.distributed_densitymatrix_expecPauliString(rho, coeffs, paulis)
+    t_exp = (time.perf_counter() - t0) / reps
+    fsa.distributed_densitymatrix_partialTrace(rho, tt).close()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        red = fsa.distributed_densitymatrix_partialTrace(rho, tt)
+        red.sync()
+        red.close()
+    t_pt = (time.perf_counter() - t0) / reps
+    rho.close()
+    nbytes = sum(circ.algorithmic_bytes(nm, ar, 2 * N, dm=True) for nm, ar in ops)
+    tiles = {k: v for k, v in prof.items() if k.startswith("tile_pass")}
+    passes = sum(v[0] for v in tiles.values())
+    tile_ms = sum(v[1] for v in tiles.values())
+    pass_bytes = 2 * 16 * (1 << (2 * N))
+    return {
+        "workload": f"{N}-qubit density matrix (2^{2 * N} complex128), {len(ops)} ops "
+                    "(manyTargGate k=1-3, 1/2-qubit dephasing and depolarising, damping), "
+                    f"expecPauliString over {len(coeffs)} strings, partialTrace over {len(tt)} qubits",
+        "ops_per_s": round(len(ops) / t_ops, 1),
+        "ms_for_ops": round(t_ops * 1e3, 2),
+        "fused_passes": passes,
+        "tile_pass_GB_per_s": round(passes * pass_bytes / (tile_ms * 1e-3) / 1e9, 1) if passes else None,
+        "effective_GB_per_s": round(nbytes / t_ops / 1e9, 1),
+        "kernel_ms": {k: round(v[1], 2) for k, v in sorted(prof.items(), key=lambda kv: -kv[1][1])},
+        "expec_ms": round(t_exp * 1e3, 3),
+        "expec_us_per_term": round(t_exp * 1e6 / len(coeffs), 2),
+        "expec_value": [ex.real, ex.imag],
+        "partial_trace_ms": round(t_pt * 1e3, 3),
+    }
+
+
+def batch_bench(fsa, circ, reps=3, sample=512):
+    """4096 independent circuits (n log-uniform in [10, 20], 50-300 gates,
+    each with its own 20-string observable): every <H> and every gadget
+    gradient from one persistent-kernel graph launch, FP64 and FP32, with the
+    configuration picked by the batch autotuner on a spread sample."""
+    t0 = time.perf_counter()
+    jobs = circ.config5()
+    circuits = [fsa.Circuit(ops) for _, ops, _, _ in jobs]
+    log(f"batch: {len(jobs)} circuits generated in {time.perf_counter() - t0:.1f} s")
+    ns = np.array([j[0] for j in jobs])
+    gates = sum(len(j[1]) for j in jobs)
+    out = {"workload": f"{len(jobs)} statevectors from |0..0>, n log-uniform in [10, 20] (mean {ns.mean():.2f}), "
+                       f"{gates} gates in total (oneTarg, 1-3 controls, manyTarg k=2-3, 1-4 qubit gadgets), "
+                       f"20-string observable each; forward <H> + adjoint gradient of every gadget",
+           "circuits": len(jobs), "gates": gates}
+    res = {}
+    for prec in ("fp64", "fp32"):
+        b = fsa.Batch(prec)
+        for (n, _, coeffs, paulis), c in zip(jobs, circuits):
+            b.add(c, n, coeffs, paulis)
+        t1 = time.perf_counter()
+        tune_ms = b.autotune(sample=sample)
+        t_tune = time.perf_counter() - t1
+        b.run()                                   # build, capture, first replay
+        fsa.profile_enable(True)
+        t1 = time.perf_counter()
+        for _ in range(reps):
+            e, g = b.run()
+        dt = (time.perf_counter() - t1) / reps
+        prof = fsa.profile_read()
+        fsa.profile_enable(False)
+        kname = f"batch_{prec}"
+        kms = prof[kname][1] / prof[kname][0] if kname in prof else None
+        res[prec] = (e, np.concatenate(g))
+        out[prec] = {"circuits_per_s": round(len(jobs) / dt, 1), "ms_per_batch": round(dt * 1e3, 2),
+                     "kernel_ms": round(kms, 2) if kms else None,
+                     "gates_per_s": round(gates / dt, 1),
+                     "config": {"workgroup": b.config[0], "tile_bits": b.config[1], "low_bits": b.config[2]},
+                     "autotune_s": round(t_tune, 1), "autotune_sample_ms": round(tune_ms, 2)}
+        log(f"batch {prec}: {dt * 1e3:.1f} ms per batch, config {b.config}")
+        b.close()
+    e64, g64 = res["fp64"]
+    e32, g32 = res["fp32"]
+    out["fp32_speedup"] = round(out["fp32"]["circuits_per_s"] / out["fp64"]["circuits_per_s"], 3)
+    out["fp32_grad_rel_err_vs_fp64"] = float(np.linalg.norm(g32 - g64) / np.linalg.norm(g64))
+    out["fp32_expec_max_abs_err_vs_fp64"] = float(np.abs(e32 - e64).max())
+    return out
+
+
 def copy_calibration(n, reps=5):
     """A plain device-to-device copy of one n-qubit FP64 state (torch copy_),
     timed with events: the practical streaming ceiling next to the 8 TB/s peak,
@@ -167,6 +270,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-per-type", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-dm", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only time a plain copy of one state-sized buffer and exit")
     a = ap.parse_args()
@@ -270,6 +375,12 @@ def main():
     if not a.no_adjoint:
         log("adjoint gradient, FP64 and FP32-bulk")
         result["adjoint"] = adjoint_bench(fsa, circ)
+    if not a.no_dm:
+        log("density matrix: 400 ops, expecPauliString, partialTrace")
+        result["density_matrix"] = dm_bench(fsa, circ)
+    if not a.no_batch:
+        log("batch: 4096 circuits with gradients")
+        result["batch"] = batch_bench(fsa, circ)
     if not a.no_per_type:
         log("per-gate-type eager timing")
         result["per_gate_type"] = per_gate_type(fsa, circ, n, ops)

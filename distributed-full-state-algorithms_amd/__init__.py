@@ -431,6 +431,79 @@ def adjoint_gradient(circuit: Circuit, psi0: StateVector, coeffs, paulis):
     return expec.value, grads
 
 
+class Batch:
+    """Many independent circuits, each on its own statevector from |0..0> with
+    its own observable; ``run()`` returns every <H> and every gadget gradient
+    from one persistent-kernel launch (C ABI fsa_batch_*)."""
+
+    def __init__(self, precision="fp64"):
+        self._h = C.c_void_p()
+        _check(lib().fsa_batch_create(_i(_precision(precision)), C.byref(self._h)))
+        self.sizes = []            # parameter count per circuit
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            lib().fsa_batch_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add(self, circuit: Circuit, numQubits: int, coeffs, paulis) -> int:
+        c = np.ascontiguousarray(np.asarray(coeffs, dtype=np.float64))
+        p = _nats(np.asarray(paulis).ravel())
+        if p.size != c.size * numQubits:
+            raise ValueError("paulis must hold numTerms * numQubits codes")
+        idx = _u()
+        _check(lib().fsa_batch_add(self._h, circuit._h, _u(numQubits), _ptr(c), _u(c.size), _ptr(p), C.byref(idx)))
+        self.sizes.append(circuit.num_params)
+        return int(idx.value)
+
+    @property
+    def num_circuits(self) -> int:
+        n, p = _u(), _u()
+        _check(lib().fsa_batch_info(self._h, C.byref(n), C.byref(p)))
+        return int(n.value)
+
+    @property
+    def num_params(self) -> int:
+        n, p = _u(), _u()
+        _check(lib().fsa_batch_info(self._h, C.byref(n), C.byref(p)))
+        return int(p.value)
+
+    def set_params(self, index: int, thetas):
+        t = np.ascontiguousarray(np.asarray(thetas, dtype=np.float64))
+        _check(lib().fsa_batch_set_params(self._h, _u(index), _ptr(t), _u(t.size)))
+
+    def set_config(self, workgroup_size: int, tile_bits: int, low_bits: int):
+        _check(lib().fsa_batch_set_config(self._h, _i(workgroup_size), _i(tile_bits), _i(low_bits)))
+
+    @property
+    def config(self):
+        a, b, c = _i(), _i(), _i()
+        _check(lib().fsa_batch_get_config(self._h, C.byref(a), C.byref(b), C.byref(c)))
+        return int(a.value), int(b.value), int(c.value)
+
+    def autotune(self, sample: int = 0) -> float:
+        ms = _d()
+        _check(lib().fsa_batch_autotune(self._h, _u(sample), C.byref(ms)))
+        return float(ms.value)
+
+    def run(self):
+        """(expecs[num_circuits], [grads of circuit 0, grads of circuit 1, ...])"""
+        e = np.zeros(self.num_circuits)
+        g = np.zeros(max(1, self.num_params))
+        _check(lib().fsa_batch_run(self._h, _ptr(e), _ptr(g)))
+        out, k = [], 0
+        for n in self.sizes:
+            out.append(g[k:k + n].copy())
+            k += n
+        return e, out
+
+
 class _ProfEntry(C.Structure):
     _fields_ = [("name", C.c_char * 32), ("count", C.c_uint64), ("total_ms", C.c_double)]
 

@@ -120,12 +120,12 @@ def random_mixed_state(N: int, seed: int, rank: int = 8) -> np.ndarray:
     rng = np.random.default_rng(seed)
     D = 1 << N
     w = rng.dirichlet(np.ones(rank))
-    R = np.zeros((D, D), dtype=complex)
-    for wi in w:
+    G = np.empty((rank, D), dtype=complex)
+    for i in range(rank):
         g = rng.normal(size=D) + 1j * rng.normal(size=D)
-        g /= np.linalg.norm(g)
-        R += wi * np.outer(g, g.conj())
-    return R.T.reshape(-1).copy()
+        G[i] = g / np.linalg.norm(g)
+    # column-major vector: amps[r + D c] = rho[r, c] = sum_i w_i g_i[r] conj(g_i[c])
+    return ((G.conj().T * w) @ G).reshape(-1)
 
 
 def config4(n: int = 24, layers: int = 20, seed: int = 20261206, num_terms: int = 100):
@@ -147,6 +147,58 @@ def config4(n: int = 24, layers: int = 20, seed: int = 20261206, num_terms: int 
         paulis[t, qs] = rng.integers(1, 4, w)
     coeffs = rng.normal(size=num_terms)
     return ops, coeffs, paulis
+
+
+def haar_stack(count: int, d: int, rng: np.random.Generator) -> np.ndarray:
+    """count independent Haar U(d) matrices (the law of haar(), vectorised)."""
+    z = (rng.normal(size=(count, d, d)) + 1j * rng.normal(size=(count, d, d))) / np.sqrt(2)
+    q, r = np.linalg.qr(z)
+    dg = np.diagonal(r, axis1=1, axis2=2)
+    return q * (dg / np.abs(dg))[:, None, :]
+
+
+def config5(seed: int = 20261207, num: int = 4096, nmin: int = 10, nmax: int = 20, num_terms: int = 20):
+    """Batch of independent circuits: n = round(exp(U(ln nmin, ln nmax))),
+    50-300 gates each, uniform over oneTargGate (Haar 2x2), manyCtrlOneTargGate
+    (1-3 controls, Haar 2x2), manyTargGate (k = 2 or 3, Haar) and pauliGadget
+    (1-4 qubits, Paulis in {X, Y, Z}, theta ~ U(0, 2pi)); each circuit has its
+    own num_terms-string observable (weight 1-4, coefficients N(0,1)).
+    Returns [(n, ops, coeffs, paulis)]."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(num):
+        n = int(round(np.exp(rng.uniform(np.log(nmin), np.log(nmax)))))
+        g = int(rng.integers(50, 301))
+        kinds = rng.integers(0, 4, g)
+        ks = rng.integers(2, 4, g)                     # manyTargGate sizes
+        u2 = haar_stack(g, 2, rng)
+        u4 = haar_stack(g, 4, rng)
+        u8 = haar_stack(g, 8, rng)
+        perms = np.argsort(rng.random((g, n)), axis=1)  # distinct qubits per op
+        ctrl = rng.integers(1, 4, g)
+        wid = rng.integers(1, 5, g)
+        codes = rng.integers(1, 4, (g, 4))
+        angles = rng.uniform(0, TWO_PI, g)
+        ops = []
+        for i in range(g):
+            q = [int(x) for x in perms[i]]
+            k = int(kinds[i])
+            if k == 0:
+                ops.append(("oneTargGate", (q[0], u2[i])))
+            elif k == 1:
+                c = min(int(ctrl[i]), n - 1)
+                ops.append(("manyCtrlOneTargGate", (q[1:1 + c], q[0], u2[i])))
+            elif k == 2:
+                ops.append(("manyTargGate", (q[:int(ks[i])], u4[i] if ks[i] == 2 else u8[i])))
+            else:
+                w = int(wid[i])
+                ops.append(("pauliGadget", (q[:w], [int(x) for x in codes[i, :w]], float(angles[i]))))
+        paulis = np.zeros((num_terms, n), dtype=np.uint32)
+        for t in range(num_terms):
+            w = int(rng.integers(1, 5))
+            paulis[t, _qubits(rng, n, w)] = rng.integers(1, 4, w)
+        out.append((n, ops, rng.normal(size=num_terms), paulis))
+    return out
 
 
 def algorithmic_bytes(name: str, args, nq: int, amp_bytes: int = 16, dm: bool = False) -> int:

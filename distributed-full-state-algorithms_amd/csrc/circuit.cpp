@@ -14,6 +14,29 @@
 
 using namespace fsa;
 
+namespace fsa {
+
+HOp inverse(const HOp& op) {
+    HOp v = op;
+    switch (op.kind) {
+    case OP_MAT1:
+    case OP_MATK: {
+        uint64_t d = op.kind == OP_MAT1 ? 2 : (1ULL << op.q.size());
+        for (uint64_t r = 0; r < d; r++)
+            for (uint64_t k = 0; k < d; k++) v.mat[r * d + k] = std::conj(op.mat[k * d + r]);
+        break;
+    }
+    case OP_PAULI:
+        if (op.gadget) { v.theta = -op.theta; refresh_angle(v); }
+        break;
+    case OP_PHASE: v.theta = -op.theta; break;
+    default: break;
+    }
+    return v;
+}
+
+}  // namespace fsa
+
 namespace {
 
 
@@ -75,24 +98,6 @@ const Plan& plan_for(Circuit& c, unsigned nq, int prec) {
     return c.cached;
 }
 
-HOp inverse(const HOp& op) {
-    HOp v = op;
-    switch (op.kind) {
-    case OP_MAT1:
-    case OP_MATK: {
-        uint64_t d = op.kind == OP_MAT1 ? 2 : (1ULL << op.q.size());
-        for (uint64_t r = 0; r < d; r++)
-            for (uint64_t k = 0; k < d; k++) v.mat[r * d + k] = std::conj(op.mat[k * d + r]);
-        break;
-    }
-    case OP_PAULI:
-        if (op.gadget) { v.theta = -op.theta; refresh_angle(v); }
-        break;
-    case OP_PHASE: v.theta = -op.theta; break;
-    default: break;
-    }
-    return v;
-}
 
 // a statevector scratch state on a borrowed stream
 struct Scratch {

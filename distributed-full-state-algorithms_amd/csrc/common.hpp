@@ -87,6 +87,7 @@ enum OpKind : uint32_t {
     OP_DEPOL2 = 9,
     OP_DAMP = 10,
     OP_SWEEP = 11,   // tile pass item: a run of ops applied to register groups (see kernels_tile.hip)
+    OP_GRAD = 12,    // batched adjoint: accumulate <lambda|P|phi> over the tile (see kernels_batch.hip)
 };
 
 // One item of a fused tile pass, or one micro-op inside a sweep item.
@@ -121,6 +122,30 @@ struct TileGeom {
     uint8_t m;            // number of tile qubits (tile = 2^m amplitudes)
     uint8_t pad;
     uint64_t tileMask;    // OR of 1 << q[i]
+};
+
+// ---- batched circuits (kernels_batch.hip) -------------------------------------
+// One job = one circuit on its own n-qubit statevector, started from |0..0>:
+// forward plan, lambda = H phi and <H> = <phi|lambda>, then the reverse plan
+// (inverse gates on phi and lambda, OP_GRAD items between them).
+struct BatchStage {
+    uint64_t tileMask;    // global positions of the m tile qubits
+    uint32_t m;           // tile bits
+    uint32_t item0;       // first item (index into the item array)
+    uint32_t nitems;
+    uint32_t pad;
+};
+
+struct BatchJob {
+    uint64_t phi, lam;    // element offsets of phi / lambda in the arena (unused when resident)
+    uint32_t n;           // qubits
+    uint32_t resident;    // 1: phi and lambda stay in LDS for the whole job (n <= tile bits)
+    uint32_t fwd0, nfwd;  // forward stages
+    uint32_t rev0, nrev;  // reverse stages
+    uint32_t term0, nterm;// observable terms (PauliTermDev)
+    uint32_t out;         // result slot of <H> (two doubles)
+    uint32_t slot0, nslot;// result slots of the job's gradients, zeroed at the start
+    uint32_t pad;
 };
 
 }  // namespace fsa

@@ -39,7 +39,8 @@ void finish_op(HOp& op) {
     case OP_MAT1: op.nd = m; op.touch = m | op.ctrl; op.tileable = true; break;
     case OP_MATK: op.nd = m; op.touch = m; op.tileable = op.q.size() <= 5; break;
     case OP_SWAP: op.nd = m; op.touch = m; op.tileable = true; break;
-    case OP_PAULI: op.nd = op.mxy; op.touch = op.mxy | op.myz; op.tileable = true; break;
+    case OP_PAULI:
+    case OP_GRAD: op.nd = op.mxy; op.touch = op.mxy | op.myz; op.tileable = true; break;
     case OP_PHASE: op.nd = 0; op.touch = op.pmask; op.tileable = true; break;
     case OP_DEPHASE1:
     case OP_DEPHASE2: op.nd = 0; op.touch = m; op.tileable = true; break;
@@ -273,22 +274,6 @@ hipEvent_t prof_event() {
     return e;
 }
 
-// brackets one launch on `stream` when timing is enabled
-struct ProfScope {
-    ProfRecord rec{nullptr, nullptr, nullptr};
-    hipStream_t stream;
-    ProfScope(const char* name, hipStream_t s) : stream(s) {
-        if (!g_prof) return;
-        rec = ProfRecord{name, prof_event(), prof_event()};
-        hip_check(hipEventRecord(rec.start, stream), "hipEventRecord");
-    }
-    ~ProfScope() {
-        if (!rec.name) return;
-        (void)hipEventRecord(rec.stop, stream);
-        g_records.push_back(rec);
-    }
-};
-
 const char* op_name(OpKind k) {
     switch (k) {
     case OP_MAT1: return "mat1";
@@ -301,11 +286,26 @@ const char* op_name(OpKind k) {
     case OP_DEPOL1: return "depol1";
     case OP_DEPOL2: return "depol2";
     case OP_DAMP: return "damp";
+    default: break;
     }
     return "op";
 }
 
 }  // namespace
+
+ProfScope::ProfScope(const char* name, hipStream_t s) : stream_(s) {
+    if (!g_prof) return;
+    name_ = name;
+    start_ = prof_event();
+    stop_ = prof_event();
+    hip_check(hipEventRecord(start_, stream_), "hipEventRecord");
+}
+
+ProfScope::~ProfScope() {
+    if (!name_) return;
+    (void)hipEventRecord(stop_, stream_);
+    g_records.push_back(ProfRecord{name_, start_, stop_});
+}
 
 void profile_enable(bool on) {
     for (auto& r : g_records) { g_event_pool.push_back(r.start); g_event_pool.push_back(r.stop); }
@@ -391,6 +391,12 @@ TileOp translate(const HOp& op, const int* pos, uint64_t tileMask, std::vector<c
         t.gmask = op.myz & ~tileMask;
         t.f[0] = op.th.real(); t.f[1] = op.th.imag();
         t.f[2] = op.fe.real(); t.f[3] = op.fe.imag();
+        break;
+    case OP_GRAD:                       // <lambda|P|phi> tile sum; the caller sets the result slot
+        t.lmask = local_mask(op.mxy);
+        t.lmask2 = local_mask(op.myz);
+        t.gmask = op.myz & ~tileMask;
+        t.f[0] = op.pw.real(); t.f[1] = op.pw.imag();
         break;
     case OP_PHASE:
         t.lmask = local_mask(op.pmask);
@@ -738,6 +744,13 @@ void execute_impl(State& s, const std::vector<HOp>& ops, const std::vector<Stage
 }
 
 }  // namespace
+
+template <typename R>
+TileOp translate_item(const HOp& op, const int* pos, uint64_t tileMask, std::vector<cx<R>>& mats) {
+    return translate<R>(op, pos, tileMask, mats, false);
+}
+template TileOp translate_item<double>(const HOp&, const int*, uint64_t, std::vector<cx<double>>&);
+template TileOp translate_item<float>(const HOp&, const int*, uint64_t, std::vector<cx<float>>&);
 
 void execute(State& s, const std::vector<HOp>& ops, const std::vector<Stage>& stages) {
     if (s.prec == FSA_FP64) execute_impl<double>(s, ops, stages);

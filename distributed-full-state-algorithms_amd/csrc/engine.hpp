@@ -45,6 +45,9 @@ struct HOp {
     bool tileable = true;
 };
 
+// the inverse of a unitary op (conjugate transpose; gadget angles negated)
+HOp inverse(const HOp& op);
+
 // recompute PAULI factors / PHASE factors from theta (gadget parameters)
 void refresh_angle(HOp& op);
 void finish_op(HOp& op);            // fills nd / touch / tileable
@@ -115,6 +118,25 @@ void execute(State& s, const std::vector<HOp>& ops, const std::vector<Stage>& st
 inline void execute(State& s, const Plan& p) { execute(s, p.ops, p.stages); }
 void execute_now(State& s, const std::vector<HOp>& ops);
 void flush(State& s);
+
+// One stage op as a tile item (VALU form for dense gates); pos[q] = tile
+// position of qubit q or -1; dense matrices are appended to mats.
+template <typename R>
+TileOp translate_item(const HOp& op, const int* pos, uint64_t tileMask, std::vector<cx<R>>& mats);
+
+// brackets launches on `stream` with HIP events while profiling is enabled
+class ProfScope {
+public:
+    ProfScope(const char* name, hipStream_t s);
+    ~ProfScope();
+    ProfScope(const ProfScope&) = delete;
+    ProfScope& operator=(const ProfScope&) = delete;
+
+private:
+    const char* name_ = nullptr;
+    hipEvent_t start_ = nullptr, stop_ = nullptr;
+    hipStream_t stream_;
+};
 
 struct ProfEntry {
     std::string name;

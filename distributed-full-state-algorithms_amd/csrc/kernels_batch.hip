@@ -1,0 +1,303 @@
+// kernels_batch.hip -- many independent circuits with their adjoint gradients
+// in one persistent kernel.
+//
+// Each workgroup takes whole jobs (one circuit on one statevector) from a
+// global counter, largest first (the host sorts them), and runs the job alone:
+//   1. forward plan from |0..0>: fused passes over LDS tiles of phi;
+//   2. lambda = H phi and <H> = <phi|lambda> (FP64 sums);
+//   3. reverse plan: the inverse gates applied to phi and lambda tile by tile,
+//      with OP_GRAD items accumulating <lambda|P_k|phi> in front of the
+//      inverse of gadget k (d<H>/d theta_k = -2 Im of it for exp(i theta P)).
+// A state of at most M qubits lives in LDS for the whole job (no HBM traffic
+// at all); a larger one lives in the arena and every pass streams it through
+// LDS one tile at a time.  One workgroup owns a job from start to end, so the
+// only synchronisation is the workgroup barrier, every sum has a fixed order
+// (results are bit-identical run to run) and there is no inter-workgroup wait:
+// each workgroup leaves the loop as soon as the counter passes the last job.
+//
+// Gadget angles are read from `params` at run time (item field param, sign in
+// f[6], i^#Y in f[4..5]), so new angles only need a copy into that array:
+// the launch itself can be captured once in a graph and replayed.
+#include "launch.hpp"
+#include "tile_ops.hpp"
+
+namespace fsa {
+
+namespace {
+
+using namespace tile;
+
+// deposit the low bits of x into the set bits of mask, ascending
+__device__ __forceinline__ uint64_t pdep64(uint64_t x, uint64_t mask) {
+    uint64_t r = 0;
+    for (; mask && x; mask &= mask - 1, x >>= 1)
+        if (x & 1ULL) r |= mask & (~mask + 1ULL);
+    return r;
+}
+
+__device__ __forceinline__ double bwave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// workgroup sum of (vr, vi); the total is returned to every thread
+template <int TB>
+__device__ __forceinline__ void bsum2(double& vr, double& vi, double* red) {
+    vr = bwave_sum(vr);
+    vi = bwave_sum(vi);
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    __syncthreads();                       // red may still be read by a previous sum
+    if (lane == 0) { red[2 * wave] = vr; red[2 * wave + 1] = vi; }
+    __syncthreads();
+    double a = 0, b = 0;
+#pragma unroll
+    for (int w = 0; w < TB / 64; w++) { a += red[2 * w]; b += red[2 * w + 1]; }
+    vr = a;
+    vi = b;
+}
+
+// gadget factors from the run-time angle (forward: sign +1, inverse: -1)
+__device__ __forceinline__ void bind_param(TileOp& op, const double* __restrict__ params) {
+    if (op.param == NO_PARAM) return;
+    const double t = op.f[6] * params[op.param];
+    double s, c;
+    sincos(t, &s, &c);
+    if (op.kind == OP_PAULI) {            // exp(i t P): this = cos t, other = i sin t * i^#Y
+        op.f[0] = c; op.f[1] = 0.0;
+        op.f[2] = -s * op.f[5]; op.f[3] = s * op.f[4];
+    } else {                               // OP_PHASE: e^{+i t} even parity, e^{-i t} odd
+        op.f[0] = c; op.f[1] = s;
+        op.f[2] = c; op.f[3] = -s;
+    }
+}
+
+// <lambda|P|phi> over one tile: sum_l conj(lam_l) i^#Y (-1)^{par(k & myz)} phi_k, k = l ^ mxy
+template <typename R, int TB>
+__device__ __forceinline__ void grad_partial(const cx<R>* phi, const cx<R>* lam, uint32_t T, uint64_t base,
+                                             const TileOp& op, double& vr, double& vi) {
+    const uint32_t bp = par64(base & op.gmask);
+    double ar = 0, ai = 0;
+    for (uint32_t l = threadIdx.x; l < T; l += TB) {
+        const uint32_t k = l ^ op.lmask;
+        const cx<R> x = lam[l], y = phi[k];
+        const double sg = (bp ^ par32(k & op.lmask2)) ? -1.0 : 1.0;
+        ar += sg * ((double)x.x * (double)y.x + (double)x.y * (double)y.y);
+        ai += sg * ((double)x.x * (double)y.y - (double)x.y * (double)y.x);
+    }
+    vr = op.f[0] * ar - op.f[1] * ai;
+    vi = op.f[0] * ai + op.f[1] * ar;
+}
+
+// lambda_j = sum_t f_t (-1)^{par(k & myz_t)} phi_k, k = j ^ mxy_t, for j in [j0, j0 + T);
+// accumulates <phi|lambda> over those j
+template <typename R, int TB>
+__device__ __forceinline__ void apply_terms(const cx<R>* phiAll, cx<R>* lamOut, uint64_t j0, uint32_t T,
+                                            const PauliTermDev* __restrict__ terms, uint32_t nterm,
+                                            double& er, double& ei) {
+    for (uint32_t l = threadIdx.x; l < T; l += TB) {
+        const uint64_t j = j0 + l;
+        double ar = 0, ai = 0;
+        for (uint32_t t = 0; t < nterm; t++) {
+            const PauliTermDev term = terms[t];
+            const uint64_t k = j ^ term.mxy;
+            const double sg = par64(k & term.myz) ? -1.0 : 1.0;
+            const cx<R> v = phiAll[k];
+            const double fr = sg * term.fr, fi = sg * term.fi;
+            ar += fr * (double)v.x - fi * (double)v.y;
+            ai += fr * (double)v.y + fi * (double)v.x;
+        }
+        const cx<R> p = phiAll[j];
+        er += (double)p.x * ar + (double)p.y * ai;
+        ei += (double)p.x * ai - (double)p.y * ar;
+        lamOut[l] = mk<R>((R)ar, (R)ai);
+    }
+}
+
+// the items of one stage on the LDS tile(s); OP_GRAD items add their tile sum
+// into res (thread 0, tiles in a fixed order)
+template <typename R, int TB, bool REV>
+__device__ __forceinline__ void run_items(cx<R>* phi, cx<R>* lam, uint32_t T, uint64_t base,
+                                          const TileOp* __restrict__ items, uint32_t i0, uint32_t ni,
+                                          const cx<R>* __restrict__ mats, const double* __restrict__ params,
+                                          double* __restrict__ res, double* red, bool firstTile) {
+    for (uint32_t i = i0; i < i0 + ni; i++) {
+        TileOp op = items[i];
+        if (REV && op.kind == OP_GRAD) {
+            double vr, vi;
+            grad_partial<R, TB>(phi, lam, T, base, op, vr, vi);
+            bsum2<TB>(vr, vi, red);
+            if (threadIdx.x == 0) {
+                double* r = res + 2 * (size_t)op.mat_off;
+                if (firstTile) { r[0] = vr; r[1] = vi; }
+                else { r[0] += vr; r[1] += vi; }
+            }
+            continue;                       // no barrier needed: nothing was written to LDS
+        }
+        bind_param(op, params);
+        tile_apply<R, TB, 1>(phi, T, base, op, mats);
+        if (REV) tile_apply<R, TB, 1>(lam, T, base, op, mats);
+        __syncthreads();
+    }
+}
+
+template <typename R, int TB, int M>
+__global__ __launch_bounds__(TB) void k_batch(const BatchJob* __restrict__ jobs, unsigned numJobs,
+                                              const BatchStage* __restrict__ stages,
+                                              const TileOp* __restrict__ items, const cx<R>* __restrict__ mats,
+                                              const PauliTermDev* __restrict__ terms,
+                                              const double* __restrict__ params, cx<R>* __restrict__ arena,
+                                              double* __restrict__ res, unsigned* __restrict__ counter) {
+    constexpr uint32_t TMAX = 1u << M;
+    constexpr uint32_t KMAX = TMAX / TB > 0 ? TMAX / TB : 1;
+    __shared__ cx<R> lds[2 * TMAX];
+    __shared__ uint64_t offHi[KMAX];
+    __shared__ double red[2 * (TB / 64)];
+    __shared__ unsigned jobIdx;
+    const uint32_t tid = threadIdx.x;
+
+    for (;;) {
+        __syncthreads();
+        if (tid == 0) jobIdx = atomicAdd(counter, 1u);
+        __syncthreads();
+        const unsigned jx = jobIdx;
+        if (jx >= numJobs) break;
+        const BatchJob J = jobs[jx];
+
+        if (J.resident) {
+            // ---- whole job in LDS: phi = lds[0, T), lambda = lds[T, 2T)
+            const uint32_t T = 1u << J.n;
+            cx<R>* phi = lds;
+            cx<R>* lam = lds + T;
+            for (uint32_t l = tid; l < T; l += TB) phi[l] = mk<R>(l == 0 ? (R)1 : (R)0, (R)0);
+            __syncthreads();
+            for (uint32_t s = J.fwd0; s < J.fwd0 + J.nfwd; s++)
+                run_items<R, TB, false>(phi, lam, T, 0, items, stages[s].item0, stages[s].nitems, mats, params, res,
+                                        red, true);
+            double er = 0, ei = 0;
+            apply_terms<R, TB>(phi, lam, 0, T, terms + J.term0, J.nterm, er, ei);
+            bsum2<TB>(er, ei, red);
+            if (tid == 0) { res[2 * (size_t)J.out] = er; res[2 * (size_t)J.out + 1] = ei; }
+            __syncthreads();
+            for (uint32_t s = J.rev0; s < J.rev0 + J.nrev; s++)
+                run_items<R, TB, true>(phi, lam, T, 0, items, stages[s].item0, stages[s].nitems, mats, params, res,
+                                       red, true);
+            continue;
+        }
+
+        // ---- state in the arena, streamed through LDS tile by tile
+        cx<R>* gphi = arena + J.phi;
+        cx<R>* glam = arena + J.lam;
+        const uint64_t all = (J.n >= 64) ? ~0ULL : ((1ULL << J.n) - 1ULL);
+        for (int dir = 0; dir < 2; dir++) {
+            const uint32_t s0 = dir == 0 ? J.fwd0 : J.rev0;
+            const uint32_t ns = dir == 0 ? J.nfwd : J.nrev;
+            for (uint32_t s = s0; s < s0 + ns; s++) {
+                const BatchStage st = stages[s];
+                const uint32_t T = 1u << st.m;
+                const uint32_t K = T > TB ? T / TB : 1u;
+                const uint64_t outMask = all & ~st.tileMask;
+                const uint64_t numTiles = 1ULL << (J.n - st.m);
+                __syncthreads();
+                if (tid < K) offHi[tid] = pdep64((uint64_t)tid * TB, st.tileMask);
+                const uint64_t offLo = pdep64(tid, st.tileMask);
+                __syncthreads();
+                const bool fromZero = dir == 0 && s == s0;   // the first pass starts from |0..0>
+                for (uint64_t t = 0; t < numTiles; t++) {
+                    const uint64_t base = pdep64(t, outMask);
+                    for (uint32_t k = 0; k < K; k++) {
+                        const uint32_t l = tid + k * TB;
+                        if (l >= T) break;
+                        const uint64_t g = base | offLo | offHi[k];
+                        lds[l] = fromZero ? mk<R>(g == 0 ? (R)1 : (R)0, (R)0) : gphi[g];
+                        if (dir == 1) lds[TMAX + l] = glam[g];
+                    }
+                    __syncthreads();
+                    if (dir == 0)
+                        run_items<R, TB, false>(lds, lds + TMAX, T, base, items, st.item0, st.nitems, mats, params,
+                                                res, red, t == 0);
+                    else
+                        run_items<R, TB, true>(lds, lds + TMAX, T, base, items, st.item0, st.nitems, mats, params,
+                                               res, red, t == 0);
+                    for (uint32_t k = 0; k < K; k++) {
+                        const uint32_t l = tid + k * TB;
+                        if (l >= T) break;
+                        const uint64_t g = base | offLo | offHi[k];
+                        gphi[g] = lds[l];
+                        if (dir == 1) glam[g] = lds[TMAX + l];
+                    }
+                    __syncthreads();
+                }
+            }
+            if (dir == 0) {
+                // the forward pass is complete in the arena (this workgroup's own
+                // stores, ordered by the barrier above): lambda = H phi, <H>
+                double er = 0, ei = 0;
+                const uint64_t num = 1ULL << J.n;
+                for (uint64_t j0 = 0; j0 < num; j0 += TMAX) {
+                    const uint32_t T = (uint32_t)((num - j0) < TMAX ? (num - j0) : TMAX);
+                    apply_terms<R, TB>(gphi, glam + j0, j0, T, terms + J.term0, J.nterm, er, ei);
+                }
+                bsum2<TB>(er, ei, red);
+                if (tid == 0) { res[2 * (size_t)J.out] = er; res[2 * (size_t)J.out + 1] = ei; }
+                __syncthreads();
+            }
+        }
+    }
+}
+
+__global__ void k_batch_reset(unsigned* counter) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) counter[0] = 0u;
+}
+
+}  // namespace
+
+template <typename R, int TB, int M>
+static const void* batch_kernel_ptr() { return reinterpret_cast<const void*>(&k_batch<R, TB, M>); }
+
+template <typename R>
+int batch_max_bits() { return sizeof(R) == 8 ? 12 : 13; }
+
+// Workgroups of one configuration that fit on a CU at once.
+template <typename R>
+int batch_occupancy(int tb, int m) {
+    int blocks = 0;
+    const void* f = nullptr;
+#define FSA_BK(TBV, MV) if (tb == TBV && m == MV) f = batch_kernel_ptr<R, TBV, MV>();
+    FSA_BK(256, 10) FSA_BK(256, 11) FSA_BK(256, 12) FSA_BK(512, 10) FSA_BK(512, 11) FSA_BK(512, 12)
+    if constexpr (sizeof(R) == 4) { FSA_BK(256, 13) FSA_BK(512, 13) }
+#undef FSA_BK
+    if (!f) return 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, f, tb, 0) != hipSuccess) return 0;
+    return blocks;
+}
+
+template <typename R>
+bool launch_batch(hipStream_t s, int tb, int m, unsigned grid, const BatchJob* jobs, unsigned numJobs,
+                  const BatchStage* stages, const TileOp* items, const cx<R>* mats, const PauliTermDev* terms,
+                  const double* params, cx<R>* arena, double* res, unsigned* counter) {
+    hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, s, counter);
+#define FSA_BL(TBV, MV)                                                                                    \
+    if (tb == TBV && m == MV) {                                                                            \
+        hipLaunchKernelGGL((k_batch<R, TBV, MV>), dim3(grid), dim3(TBV), 0, s, jobs, numJobs, stages, items, \
+                           mats, terms, params, arena, res, counter);                                      \
+        return true;                                                                                       \
+    }
+    FSA_BL(256, 10) FSA_BL(256, 11) FSA_BL(256, 12) FSA_BL(512, 10) FSA_BL(512, 11) FSA_BL(512, 12)
+    if constexpr (sizeof(R) == 4) { FSA_BL(256, 13) FSA_BL(512, 13) }
+#undef FSA_BL
+    return false;
+}
+
+template int batch_max_bits<double>();
+template int batch_max_bits<float>();
+template int batch_occupancy<double>(int, int);
+template int batch_occupancy<float>(int, int);
+template bool launch_batch<double>(hipStream_t, int, int, unsigned, const BatchJob*, unsigned, const BatchStage*,
+                                   const TileOp*, const cx<double>*, const PauliTermDev*, const double*,
+                                   cx<double>*, double*, unsigned*);
+template bool launch_batch<float>(hipStream_t, int, int, unsigned, const BatchJob*, unsigned, const BatchStage*,
+                                  const TileOp*, const cx<float>*, const PauliTermDev*, const double*, cx<float>*,
+                                  double*, unsigned*);
+
+}  // namespace fsa

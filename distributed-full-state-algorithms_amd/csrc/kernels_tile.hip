@@ -9,6 +9,7 @@
 // are constant over a workgroup and are folded into a per-tile test or sign.
 // The arithmetic per amplitude is the same as the streaming kernels'.
 #include "launch.hpp"
+#include "tile_ops.hpp"
 
 #include <algorithm>
 
@@ -16,356 +17,7 @@ namespace fsa {
 
 namespace {
 
-// bit value of a channel qubit: code bit 8 set -> local position in the tile,
-// otherwise a global qubit index outside the tile (fixed for the workgroup)
-__device__ __forceinline__ uint32_t chan_bit(uint32_t l, uint64_t base, uint32_t code) {
-    return (code & 0x100u) ? (l >> (code & 0xffu)) & 1u : (uint32_t)((base >> code) & 1ULL);
-}
-
-template <typename R>
-__device__ __forceinline__ cx<R> ldc(const double* f, int i) { return mk<R>((R)f[2 * i], (R)f[2 * i + 1]); }
-
-// dense k-target gate on LDS.  When the tile holds at least TB groups each
-// thread owns whole groups (in place).  Otherwise the 2^K outputs of a group
-// are split over TB / groups threads: every thread streams the group's inputs
-// from LDS, accumulates its outputs in registers, and writes them after a
-// barrier (so no thread overwrites inputs another still reads).  Outputs
-// accumulate in input order, as in the streaming kernel.
-// tile offset of matrix index l: bit i of l -> tile bit p_i (bits packed 5 per position)
-template <int K>
-__device__ __forceinline__ uint32_t mat_offset(uint32_t l, uint32_t b0, uint32_t b1, uint32_t b2, uint32_t b3,
-                                               uint32_t b4) {
-    uint32_t o = (l & 1u) ? b0 : 0u;
-    if (K > 1 && (l & 2u)) o |= b1;
-    if (K > 2 && (l & 4u)) o |= b2;
-    if (K > 3 && (l & 8u)) o |= b3;
-    if (K > 4 && (l & 16u)) o |= b4;
-    return o;
-}
-
-template <typename R, int K, int TB>
-__device__ __forceinline__ void tile_matk(cx<R>* s, uint32_t T, const TileOp& op, const cx<R>* __restrict__ mat) {
-    constexpr int D = 1 << K;
-    const uint32_t b0 = 1u << op.p[0];
-    const uint32_t b1 = K > 1 ? 1u << op.p[1] : 0u;
-    const uint32_t b2 = K > 2 ? 1u << op.p[2] : 0u;
-    const uint32_t b3 = K > 3 ? 1u << op.p[3] : 0u;
-    const uint32_t b4 = K > 4 ? 1u << op.p[4] : 0u;
-    const uint32_t tm = b0 | b1 | b2 | b3 | b4;
-#define FSA_OFF(l) mat_offset<K>((l), b0, b1, b2, b3, b4)
-    const uint32_t groups = T >> K;
-    const uint32_t tid = threadIdx.x;
-    constexpr int OUTMAX = sizeof(R) == 8 ? 8 : 16;
-    constexpr int OUT = D < OUTMAX ? D : OUTMAX;
-    // whole groups per thread, written in place; a full tile has >= TB groups only when D <= OUT
-    if constexpr (D <= OUT) if (groups >= (uint32_t)TB) {
-        for (uint32_t g = tid; g < groups; g += TB) {
-            const cx<R>* m = mat;
-            const uint32_t b = ins0_mask32(g, tm);
-            cx<R> acc[D];
-#pragma unroll
-            for (int o = 0; o < D; o++) acc[o] = mk<R>(0, 0);
-            // input loop kept rolled: D accumulators and one matrix column live at a time
-#pragma unroll 1
-            for (int l = 0; l < D; l++) {
-                cx<R> x = s[b | FSA_OFF((uint32_t)l)];
-#pragma unroll
-                for (int o = 0; o < D; o++) acc[o] = cfma(acc[o], m[o * D + l], x);
-            }
-#pragma unroll
-            for (int o = 0; o < D; o++) s[b | FSA_OFF((uint32_t)o)] = acc[o];
-        }
-        return;
-    }
-    // split mode (one group per thread, tpg threads per group, outs consecutive
-    // outputs each): all inputs are read before the barrier, outputs written after
-    const uint32_t byThreads = TB / groups;
-    const uint32_t tpg = byThreads < (uint32_t)(D / OUT) ? byThreads : (uint32_t)(D / OUT);
-    const uint32_t outs = D / tpg;
-    const uint32_t g = tid & (groups - 1), sub = tid / groups;
-    const bool active = sub < tpg;
-    const uint32_t r0 = sub * outs;
-    const uint32_t b = ins0_mask32(g, tm);
-    cx<R> acc[OUT];
-#pragma unroll
-    for (int o = 0; o < OUT; o++) acc[o] = mk<R>(0, 0);
-    if (groups >= 64 && outs == (uint32_t)OUT) {
-        // a wave spans one `sub`, so its output rows are wave-uniform: read the
-        // matrix through the scalar unit instead of one vector load per lane
-        const uint32_t r0u = __builtin_amdgcn_readfirstlane(r0);
-        const cx<R>* mrow = mat + (size_t)r0u * D;
-        if (active) {
-#pragma unroll 1
-            for (int l = 0; l < D; l++) {
-                cx<R> x = s[b | FSA_OFF((uint32_t)l)];
-#pragma unroll
-                for (int o = 0; o < OUT; o++) acc[o] = cfma(acc[o], mrow[o * D + l], x);
-            }
-        }
-    } else if (active) {
-#pragma unroll 1
-        for (int l = 0; l < D; l++) {
-            cx<R> x = s[b | FSA_OFF((uint32_t)l)];
-#pragma unroll
-            for (int o = 0; o < OUT; o++)
-                if ((uint32_t)o < outs) acc[o] = cfma(acc[o], mat[(r0 + o) * D + l], x);
-        }
-    }
-    __syncthreads();
-    if (active) {
-#pragma unroll
-        for (int o = 0; o < OUT; o++)
-            if ((uint32_t)o < outs) s[b | FSA_OFF(r0 + o)] = acc[o];
-    }
-#undef FSA_OFF
-}
-
-// ---- dense gates on MFMA ----------------------------------------------------
-// Y = U X for the 2^K amplitudes of every group in the tile, as the real GEMM
-// [Yr; Yi] = [[Ur, -Ui], [Ui, Ur]] [Xr; Xi] (2D x 2D times 2D x groups, D = 2^K)
-// on 16x16x4 MFMA tiles.  Operand layout of v_mfma_*_16x16x4: lane l holds
-// A[l % 16][l / 16] and B[l / 16][l % 16]; accumulator element j of lane l is
-// column l % 16, row 4 j + l / 16 for f64 and row 4 (l / 16) + j for f32
-// (measured on gfx950).  A wave task is one 16-row block
-// of the real matrix against four 16-group column blocks; the real matrix (in
-// the parameter buffer, row-major 2D x 2D) feeds A, the tile in LDS feeds B.
-// All inputs are read before the barrier, outputs written after it.
-typedef double fsa_d4 __attribute__((ext_vector_type(4)));
-typedef float fsa_f4 __attribute__((ext_vector_type(4)));
-template <typename R> struct MfmaAcc;
-template <> struct MfmaAcc<double> {
-    typedef fsa_d4 T;
-    static __device__ __forceinline__ T step(double a, double b, T c) {
-        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
-    }
-};
-template <> struct MfmaAcc<float> {
-    typedef fsa_f4 T;
-    static __device__ __forceinline__ T step(float a, float b, T c) {
-        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
-    }
-};
-
-// tasks per wave for a full tile: FP64 (4096 amplitudes) 1, FP32 (8192) 2
-template <typename R, int K, int TB>
-__device__ __forceinline__ void tile_matk_mfma(cx<R>* s, uint32_t T, const TileOp& op, const R* __restrict__ M) {
-    typedef typename MfmaAcc<R>::T acc_t;
-    constexpr int D = 1 << K, D2 = 2 * D;
-    constexpr int RB = D2 / 16;                 // 16-row blocks of the real matrix
-    constexpr int WAVES = TB / 64;
-    constexpr int MAXT = sizeof(R) == 8 ? 1 : 2;
-    const uint32_t b0 = 1u << op.p[0], b1 = 1u << op.p[1], b2 = 1u << op.p[2];
-    const uint32_t b3 = K > 3 ? 1u << op.p[3] : 0u;
-    const uint32_t b4 = K > 4 ? 1u << op.p[4] : 0u;
-    const uint32_t tm = b0 | b1 | b2 | b3 | b4;
-    const uint32_t groups = T >> K;
-    const uint32_t cb4 = groups / 64;          // groups of four 16-column blocks
-    const uint32_t tasks = RB * cb4;
-    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-    const uint32_t lr = lane & 15u, lk = lane >> 4;
-    acc_t acc[MAXT][4];
-    uint32_t taskId[MAXT];
-#pragma unroll
-    for (int t = 0; t < MAXT; t++) {
-        taskId[t] = wave + (uint32_t)t * WAVES;
-#pragma unroll
-        for (int c = 0; c < 4; c++) acc[t][c] = acc_t{0, 0, 0, 0};
-        if (taskId[t] >= tasks) continue;
-        const uint32_t rb = taskId[t] % RB, cbase = (taskId[t] / RB) * 4;
-        const R* arow = M + (size_t)(rb * 16 + lr) * D2;
-        // B-fragment group bases of the four column blocks for this lane
-        uint32_t gb[4];
-#pragma unroll
-        for (int c = 0; c < 4; c++) gb[c] = ins0_mask32((cbase + (uint32_t)c) * 16 + lr, tm);
-        const R* sr = reinterpret_cast<const R*>(s);
-        // four k-steps per iteration: their A loads and LDS reads issue together
-#pragma unroll 4
-        for (int ks = 0; ks < D2 / 4; ks++) {
-            const R a = arow[ks * 4 + lk];
-            const uint32_t r = (uint32_t)ks * 4 + lk;          // real row of X this lane feeds
-            const uint32_t off = mat_offset<K>(r & (D - 1), b0, b1, b2, b3, b4);
-            const uint32_t half = r < (uint32_t)D ? 0u : 1u;  // real or imaginary part only
-#pragma unroll
-            for (int c = 0; c < 4; c++) acc[t][c] = MfmaAcc<R>::step(a, sr[2 * (gb[c] | off) + half], acc[t][c]);
-        }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int t = 0; t < MAXT; t++) {
-        if (taskId[t] >= tasks) continue;
-        const uint32_t rb = taskId[t] % RB, cbase = (taskId[t] / RB) * 4;
-#pragma unroll
-        for (int c = 0; c < 4; c++) {
-            const uint32_t gbc = ins0_mask32((cbase + (uint32_t)c) * 16 + lr, tm);
-#pragma unroll
-            for (int j = 0; j < 4; j++) {
-                // accumulator row j of this lane: FP64 4j + lane/16, FP32 4 (lane/16) + j
-                const uint32_t row = rb * 16 + (sizeof(R) == 8 ? 4u * (uint32_t)j + lk : lk * 4u + (uint32_t)j);
-                const uint32_t l = row & (D - 1);
-                R* dst = reinterpret_cast<R*>(&s[gbc | mat_offset<K>(l, b0, b1, b2, b3, b4)]);
-                dst[row < (uint32_t)D ? 0 : 1] = acc[t][c][j];
-            }
-        }
-    }
-}
-
-// Kernel variants: VAR 0 carries the simple LDS ops (and, with SW, register
-// sweeps); VAR 1 adds VALU dense gates and two-qubit depolarising; VAR 2 has
-// MFMA dense gates instead of VALU ones.  Each instantiation stays within 128
-// VGPRs so two 64 KiB workgroups share a CU.
-template <typename R, int TB, int VAR>
-__device__ __forceinline__ void tile_apply(cx<R>* s, uint32_t T, uint64_t base, const TileOp& op,
-                           const cx<R>* __restrict__ mats) {
-    const uint32_t tid = threadIdx.x;
-    switch (op.kind) {
-    case OP_MAT1: {
-        if ((base & op.gmask) != op.gmask) return;           // an outside control is 0
-        const uint32_t tb = 1u << op.p[0];
-        const uint32_t lc = op.lmask;
-        const uint32_t nIts = T >> (1 + __builtin_popcount(lc));
-        const cx<R> g00 = ldc<R>(op.f, 0), g01 = ldc<R>(op.f, 1), g10 = ldc<R>(op.f, 2), g11 = ldc<R>(op.f, 3);
-        for (uint32_t j = tid; j < nIts; j += TB) {
-            uint32_t i0 = ins0_mask32(j, lc | tb) | lc;
-            cx<R> v0 = s[i0], v1 = s[i0 | tb];
-            s[i0] = cfma(cmul(g00, v0), g01, v1);
-            s[i0 | tb] = cfma(cmul(g10, v0), g11, v1);
-        }
-        return;
-    }
-    case OP_SWAP: {
-        const uint32_t b0 = 1u << op.p[0], b1 = 1u << op.p[1];
-        for (uint32_t j = tid; j < (T >> 2); j += TB) {
-            uint32_t i = ins0_mask32(j, b0 | b1);
-            cx<R> x = s[i | b0], y = s[i | b1];
-            s[i | b0] = y;
-            s[i | b1] = x;
-        }
-        return;
-    }
-    case OP_PAULI: {
-        const cx<R> th = ldc<R>(op.f, 0), fe = ldc<R>(op.f, 1);
-        const cx<R> fo = mk<R>(-fe.x, -fe.y);
-        const uint32_t bp = par64(base & op.gmask);
-        const uint32_t lxy = op.lmask, lyz = op.lmask2;
-        if (lxy == 0) {
-            for (uint32_t l = tid; l < T; l += TB) {
-                cx<R> v = s[l];
-                cx<R> f = (bp ^ par32(l & lyz)) ? fo : fe;
-                s[l] = cfma(cmul(th, v), f, v);
-            }
-            return;
-        }
-        const unsigned hb = 31u - (unsigned)__builtin_clz(lxy);
-        for (uint32_t j = tid; j < (T >> 1); j += TB) {
-            uint32_t i0 = ins0_32(j, hb), i1 = i0 ^ lxy;
-            cx<R> v0 = s[i0], v1 = s[i1];
-            cx<R> f0 = (bp ^ par32(i0 & lyz)) ? fo : fe;
-            cx<R> f1 = (bp ^ par32(i1 & lyz)) ? fo : fe;
-            s[i0] = cfma(cmul(th, v0), f1, v1);
-            s[i1] = cfma(cmul(th, v1), f0, v0);
-        }
-        return;
-    }
-    case OP_PHASE: {
-        const cx<R> f0 = ldc<R>(op.f, 0), f1 = ldc<R>(op.f, 1);
-        const uint32_t bp = par64(base & op.gmask);
-        for (uint32_t l = tid; l < T; l += TB)
-            s[l] = cmul(s[l], (bp ^ par32(l & op.lmask)) ? f1 : f0);
-        return;
-    }
-    case OP_MATK: {
-        if constexpr (VAR == 0) return;
-        const cx<R>* mat = mats + op.mat_off;
-        if (op.lmask2 == 1u) {        // host chose the MFMA form (full tile, k = 3, 4 or 5)
-            if constexpr (VAR == 2) {
-                const R* M = reinterpret_cast<const R*>(mat);
-                if (op.k == 3) tile_matk_mfma<R, 3, TB>(s, T, op, M);
-                else if (op.k == 4) tile_matk_mfma<R, 4, TB>(s, T, op, M);
-                else tile_matk_mfma<R, 5, TB>(s, T, op, M);
-            }
-            return;
-        }
-        if constexpr (VAR == 2) return;
-        switch (op.k) {
-        case 1: tile_matk<R, 1, TB>(s, T, op, mat); break;
-        case 2: tile_matk<R, 2, TB>(s, T, op, mat); break;
-        case 3: tile_matk<R, 3, TB>(s, T, op, mat); break;
-        case 4: tile_matk<R, 4, TB>(s, T, op, mat); break;
-        case 5: tile_matk<R, 5, TB>(s, T, op, mat); break;
-        default: break;
-        }
-        return;
-    }
-    case OP_DEPHASE1: {
-        const R fac = (R)op.f[0];
-        const uint32_t cq = op.p[0], ca = op.p[1];
-        for (uint32_t l = tid; l < T; l += TB)
-            if (chan_bit(l, base, cq) ^ chan_bit(l, base, ca)) s[l] = cscale(s[l], fac);
-        return;
-    }
-    case OP_DEPHASE2: {
-        const R fac = (R)op.f[0];
-        for (uint32_t l = tid; l < T; l += TB) {
-            uint32_t f = (chan_bit(l, base, op.p[0]) ^ chan_bit(l, base, op.p[1])) |
-                         (chan_bit(l, base, op.p[2]) ^ chan_bit(l, base, op.p[3]));
-            if (f) s[l] = cscale(s[l], fac);
-        }
-        return;
-    }
-    case OP_DEPOL1: {
-        const R c1 = (R)op.f[0], c2 = (R)op.f[1], c3 = (R)op.f[2];
-        const uint32_t bq = 1u << op.p[0], ba = 1u << op.p[1];
-        for (uint32_t j = tid; j < (T >> 2); j += TB) {
-            uint32_t i = ins0_mask32(j, bq | ba);
-            cx<R> v00 = s[i], v01 = s[i | bq], v10 = s[i | ba], v11 = s[i | bq | ba];
-            s[i] = mk<R>(c2 * v00.x + c1 * v11.x, c2 * v00.y + c1 * v11.y);
-            s[i | bq] = cscale(v01, c3);
-            s[i | ba] = cscale(v10, c3);
-            s[i | bq | ba] = mk<R>(c1 * v00.x + c2 * v11.x, c1 * v00.y + c2 * v11.y);
-        }
-        return;
-    }
-    case OP_DAMP: {
-        const R prob = (R)op.f[0], c1 = (R)op.f[1], c2 = (R)op.f[2];
-        const uint32_t bq = 1u << op.p[0], ba = 1u << op.p[1];
-        for (uint32_t j = tid; j < (T >> 2); j += TB) {
-            uint32_t i = ins0_mask32(j, bq | ba);
-            cx<R> v00 = s[i], v01 = s[i | bq], v10 = s[i | ba], v11 = s[i | bq | ba];
-            s[i] = mk<R>(v00.x + prob * v11.x, v00.y + prob * v11.y);
-            s[i | bq] = cscale(v01, c1);
-            s[i | ba] = cscale(v10, c1);
-            s[i | bq | ba] = cscale(v11, c2);
-        }
-        return;
-    }
-    case OP_DEPOL2: {
-        if constexpr (VAR == 0) return;
-        const R c1 = (R)op.f[0], c2 = (R)op.f[1], sc = (R)op.f[2];
-        const uint32_t b0 = 1u << op.p[0], b1 = 1u << op.p[1], b2 = 1u << op.p[2], b3 = 1u << op.p[3];
-        const uint32_t mk4 = b0 | b1 | b2 | b3;
-        for (uint32_t j = tid; j < (T >> 4); j += TB) {
-            uint32_t i = ins0_mask32(j, mk4);
-            // block-diagonal quartet: row bits equal column bits on both qubits
-            const uint32_t d0 = i, d1 = i | b0 | b2, d2 = i | b1 | b3, d3 = i | mk4;
-            cx<R> v0 = s[d0], v1 = s[d1], v2 = s[d2], v3 = s[d3];
-            cx<R> sum = cadd(cadd(v0, v1), cadd(v2, v3));
-            s[d0] = mk<R>(c1 * v0.x + c2 * sum.x, c1 * v0.y + c2 * sum.y);
-            s[d1] = mk<R>(c1 * v1.x + c2 * sum.x, c1 * v1.y + c2 * sum.y);
-            s[d2] = mk<R>(c1 * v2.x + c2 * sum.x, c1 * v2.y + c2 * sum.y);
-            s[d3] = mk<R>(c1 * v3.x + c2 * sum.x, c1 * v3.y + c2 * sum.y);
-#pragma unroll
-            for (int l = 0; l < 16; l++) {
-                if (l == 0 || l == 5 || l == 10 || l == 15) continue;
-                uint32_t x = i | ((l & 1) ? b0 : 0u) | ((l & 2) ? b1 : 0u) | ((l & 4) ? b2 : 0u) | ((l & 8) ? b3 : 0u);
-                s[x] = cscale(s[x], sc);
-            }
-        }
-        return;
-    }
-    default:
-        return;
-    }
-}
+using namespace tile;
 
 // ---- register sweeps ------------------------------------------------------
 // A sweep gives each thread G = 2^W amplitudes of the tile: tile-local indices

@@ -188,6 +188,37 @@ int fsa_circuit_num_passes(fsa_circuit* c, unsigned numQubits, int precision, un
 int fsa_adjoint_gradient(fsa_circuit* c, fsa_state* psi0, const double* coeffs, unsigned numTerms,
                          const unsigned* paulis, double* expec, double* grads);
 
+/* Batches (no reference counterpart): many independent circuits, each on its
+ * own statevector started from |0..0>, each with its own observable
+ * H = sum_t coeffs[t] P_t (paulis[t*n + q] on qubit q).  fsa_batch_run
+ * computes every <H> and every gadget gradient in ONE persistent kernel
+ * launch (captured once as a HIP graph and replayed; new angles are one
+ * copy).  Amplitudes are FP64 or FP32 (precision), all sums FP64.
+ * fsa_batch_add copies the circuit's ops and current angles; the circuit may
+ * be changed or destroyed afterwards.  Circuits may hold one-target,
+ * controlled and dense (up to 5 targets) gates, swaps, Pauli tensors and
+ * gadgets; every op must fit one tile (tile bits minus low bits qubits
+ * outside the low qubits). */
+typedef struct fsa_batch fsa_batch;
+int fsa_batch_create(int precision, fsa_batch** out);
+int fsa_batch_destroy(fsa_batch* b);
+int fsa_batch_add(fsa_batch* b, const fsa_circuit* c, unsigned numQubits, const double* coeffs,
+                  unsigned numTerms, const unsigned* paulis, unsigned* index);
+int fsa_batch_info(const fsa_batch* b, unsigned* numCircuits, unsigned* numParams);
+/* new angles for circuit `index` (num = that circuit's parameter count) */
+int fsa_batch_set_params(fsa_batch* b, unsigned index, const double* thetas, unsigned num);
+/* expecs[numCircuits]; grads[numParams], circuits in add order, each in its
+ * own parameter order (either pointer may be NULL) */
+int fsa_batch_run(fsa_batch* b, double* expecs, double* grads);
+/* plan shape: workgroup size (256 or 512), tile bits (10..12 FP64, 10..13
+ * FP32; states of at most that many qubits stay in LDS for the whole job) and
+ * the number of low qubits always kept in a tile (HBM coalescing vs fusion) */
+int fsa_batch_set_config(fsa_batch* b, int workgroupSize, int tileBits, int lowBits);
+int fsa_batch_get_config(const fsa_batch* b, int* workgroupSize, int* tileBits, int* lowBits);
+/* time every configuration on a spread sample of sampleCircuits circuits
+ * (0 = all) and keep the fastest; bestMs = its time on the sample */
+int fsa_batch_autotune(fsa_batch* b, unsigned sampleCircuits, double* bestMs);
+
 /* Kernel timing: while enabled, every pass the library launches is bracketed
  * by HIP events on its stream; fsa_profile_read aggregates them per kernel
  * kind (count, total milliseconds).  Enabling clears previous records. */
