@@ -121,9 +121,10 @@ def adjoint_bench(fsa, circ, n=24, reps=3):
         fsa.profile_enable(False)
         psi0.close()
         res[prec] = (e, g)
+        top = sorted(prof.items(), key=lambda kv: -kv[1][1])[:6]
         out[prec] = {"circuits_per_s": round(1.0 / dt, 3), "ms_per_circuit": round(dt * 1e3, 2),
-                     "kernel_ms_per_circuit": {k: round(v[1] / reps, 2) for k, v in
-                                               sorted(prof.items(), key=lambda kv: -kv[1][1])[:6]}}
+                     "kernel_ms_per_circuit": {k: round(v[1] / reps, 2) for k, v in top},
+                     "launches_per_circuit": {k: round(v[0] / reps, 1) for k, v in top}}
         log(f"adjoint {prec}: {dt * 1e3:.1f} ms per circuit")
     e64, g64 = res["fp64"]
     e32, g32 = res["fp32"]

@@ -28,6 +28,8 @@ bool launch_batch(hipStream_t s, int tb, int m, unsigned grid, const BatchJob* j
                   const BatchStage* stages, const TileOp* items, const cx<R>* mats, const PauliTermDev* terms,
                   const double* params, cx<R>* arena, double* res, unsigned* counter);
 
+constexpr unsigned MAX_STAGE_GRADS_HOST = 512;   // LDS gradient accumulators per pass (kernels_batch.hip)
+
 struct BatchEntry {
     unsigned n = 0;
     std::vector<HOp> ops;
@@ -127,7 +129,7 @@ struct BlobBuilder {
 
     // ops[] with per-op angle sign (+1 forward, -1 inverse; 0 for GRAD)
     void add_plan(const std::vector<HOp>& ops, const std::vector<int>& sign, unsigned n, SchedParams sp,
-                  unsigned numJobs, uint32_t& first, uint32_t& count) {
+                  unsigned numJobs, bool rev, uint32_t& first, uint32_t& count) {
         std::vector<Stage> st = schedule(ops, n, sp);
         first = (uint32_t)stages.size();
         for (const Stage& s : st) {
@@ -142,10 +144,20 @@ struct BlobBuilder {
             }
             bs.m = (uint32_t)s.tq.size();
             bs.item0 = (uint32_t)items.size();
+            uint32_t ngrad = 0;
             for (int oi : s.ops) {
                 const HOp& op = ops[oi];
                 TileOp t = translate_item<R>(op, pos, bs.tileMask, mats);
+                if (op.kind == OP_GRAD && ngrad == MAX_STAGE_GRADS_HOST) {
+                    // close the pass and continue the same ops in a new one on the same tiles
+                    fuse_diag_runs(items, bs.item0, rev);
+                    bs.nitems = (uint32_t)(items.size() - bs.item0);
+                    stages.push_back(bs);
+                    bs.item0 = (uint32_t)items.size();
+                    ngrad = 0;
+                }
                 if (op.kind == OP_GRAD) {
+                    t.p[0] = ngrad++;
                     t.mat_off = numJobs + (uint32_t)op.param;
                     t.param = NO_PARAM;
                 } else if (op.param >= 0 && (op.kind == OP_PAULI || op.kind == OP_PHASE)) {
@@ -158,6 +170,7 @@ struct BlobBuilder {
                 }
                 items.push_back(t);
             }
+            fuse_diag_runs(items, bs.item0, rev);
             bs.nitems = (uint32_t)(items.size() - bs.item0);
             stages.push_back(bs);
         }
@@ -195,7 +208,7 @@ void build_impl(Batch& b) {
         BatchJob J{};
         J.n = e.n;
         J.resident = resident ? 1u : 0u;
-        bb.add_plan(fwd, fsign, e.n, sp, numJobs, J.fwd0, J.nfwd);
+        bb.add_plan(fwd, fsign, e.n, sp, numJobs, false, J.fwd0, J.nfwd);
         if (!resident && J.nfwd == 0) {
             // the first forward pass creates |0..0> in the arena, so there is always one
             BatchStage bs{};
@@ -206,7 +219,7 @@ void build_impl(Batch& b) {
             J.nfwd = 1;
             bb.stages.push_back(bs);
         }
-        bb.add_plan(rev, rsign, e.n, sp, numJobs, J.rev0, J.nrev);
+        bb.add_plan(rev, rsign, e.n, sp, numJobs, true, J.rev0, J.nrev);
         J.term0 = (uint32_t)bb.terms.size();
         J.nterm = (uint32_t)e.terms.size();
         bb.terms.insert(bb.terms.end(), e.terms.begin(), e.terms.end());

@@ -9,12 +9,22 @@
 #include "../../include/fsa.h"
 #include "engine.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 
 using namespace fsa;
 
 namespace fsa {
+
+template <typename R>
+int rev_tile_bits();
+template <typename R>
+unsigned rev_grid(uint64_t numTiles);
+template <typename R>
+void launch_rev_pass(hipStream_t s, unsigned grid, cx<R>* phi, cx<R>* lam, unsigned n, const BatchStage& st,
+                     const TileOp* items, const cx<R>* mats, double* part, unsigned ngrad, const uint32_t* slots,
+                     double* res);
 
 HOp inverse(const HOp& op) {
     HOp v = op;
@@ -35,6 +45,73 @@ HOp inverse(const HOp& op) {
     return v;
 }
 
+
+template <typename R>
+RevPlan<R> build_reverse_plan(const std::vector<HOp>& ops, unsigned n) {
+    RevPlan<R> p;
+    for (size_t k = ops.size(); k-- > 0;) {
+        const HOp& op = ops[k];
+        if (op.param >= 0) {
+            HOp g;
+            g.kind = OP_GRAD;
+            if (op.kind == OP_PAULI) { g.mxy = op.mxy; g.myz = op.myz; g.pw = op.pw; }
+            else { g.mxy = 0; g.myz = op.pmask; g.pw = 1.0; }
+            g.param = op.param;
+            finish_op(g);
+            p.rev.push_back(g);
+        }
+        p.rev.push_back(inverse(op));
+    }
+    SchedParams sp;
+    sp.m = std::min(n, (unsigned)rev_tile_bits<R>());
+    sp.low = n <= sp.m ? n : std::min(5u, sp.m);
+    p.m = sp.m;
+    const std::vector<Stage> stages = schedule(p.rev, n, sp);
+    for (const Stage& stg : stages) {
+        typename RevPlan<R>::Pass r{};
+        r.tiled = stg.tq.size() == sp.m;
+        if (!r.tiled) {
+            r.ops = stg.ops;
+            p.passes.push_back(r);
+            continue;
+        }
+        int pos[64];
+        for (int& x : pos) x = -1;
+        for (size_t i = 0; i < stg.tq.size(); i++) {
+            pos[stg.tq[i]] = (int)i;
+            r.bs.tileMask |= 1ULL << stg.tq[i];
+        }
+        r.bs.m = sp.m;
+        r.bs.item0 = (uint32_t)p.items.size();
+        r.slot0 = p.slots.size();
+        auto close = [&]() {
+            fuse_diag_runs(p.items, r.bs.item0, true);
+            r.bs.nitems = (uint32_t)(p.items.size() - r.bs.item0);
+            p.maxGrad = std::max(p.maxGrad, r.ngrad);
+            p.passes.push_back(r);
+        };
+        for (int oi : stg.ops) {
+            TileOp t = translate_item<R>(p.rev[oi], pos, r.bs.tileMask, p.mats);
+            t.param = NO_PARAM;                          // factors are fixed for this call
+            if (p.rev[oi].kind == OP_GRAD && r.ngrad == REV_MAX_GRADS) {
+                // close the pass and continue the same ops in a new one on the same tiles
+                close();
+                r.bs.item0 = (uint32_t)p.items.size();
+                r.slot0 = p.slots.size();
+                r.ngrad = 0;
+            }
+            if (p.rev[oi].kind == OP_GRAD) {
+                t.p[0] = r.ngrad++;
+                p.slots.push_back(1u + (uint32_t)p.rev[oi].param);
+            }
+            p.items.push_back(t);
+        }
+        close();
+    }
+    return p;
+}
+template RevPlan<double> build_reverse_plan<double>(const std::vector<HOp>&, unsigned);
+template RevPlan<float> build_reverse_plan<float>(const std::vector<HOp>&, unsigned);
 }  // namespace fsa
 
 namespace {
@@ -149,6 +226,10 @@ void adjoint_impl(Circuit& c, State& psi0, const double* coeffs, unsigned T, con
 
     // results: slot 0 = <phi|lambda>, slot 1 + p = <lambda|P_p|phi>
     const unsigned P = c.numParams;
+
+    // Reverse sweep, op by op: <lambda|P_k|phi> for each gadget, then U_k^dag
+    // on phi and on lambda.  (The tile-pass form of this sweep,
+    // build_reverse_plan + launch_rev_pass, is not used by this entry point.)
     size_t red = reduce_workspace_doubles(phi.s.num());
     double* ws = phi.s.workspace(red + 2 * (P + 1) + 8);
     double* res = ws + red;
@@ -156,12 +237,8 @@ void adjoint_impl(Circuit& c, State& psi0, const double* coeffs, unsigned T, con
     for (size_t k = c.ops.size(); k-- > 0;) {
         const HOp& op = c.ops[k];
         if (op.param >= 0) {
-            PauliTermDev term{};
-            if (op.kind == OP_PAULI) {
-                term = PauliTermDev{op.mxy, op.myz, op.pw.real(), op.pw.imag()};
-            } else {
-                term = PauliTermDev{0, op.pmask, 1.0, 0.0};
-            }
+            const PauliTermDev term = op.kind == OP_PAULI ? PauliTermDev{op.mxy, op.myz, op.pw.real(), op.pw.imag()}
+                                                          : PauliTermDev{0, op.pmask, 1.0, 0.0};
             launch_pauli_element<R>(st, a_lam, a_phi, phi.s.num(), term, ws, res + 2 * (1 + op.param));
         }
         std::vector<HOp> inv{inverse(op)};

@@ -88,6 +88,7 @@ enum OpKind : uint32_t {
     OP_DAMP = 10,
     OP_SWEEP = 11,   // tile pass item: a run of ops applied to register groups (see kernels_tile.hip)
     OP_GRAD = 12,    // batched adjoint: accumulate <lambda|P|phi> over the tile (see kernels_batch.hip)
+    OP_DIAG_RUN = 13,// header of k consecutive diagonal items applied in one LDS sweep (tile_ops.hpp)
 };
 
 // One item of a fused tile pass, or one micro-op inside a sweep item.
@@ -114,6 +115,8 @@ struct TileOp {
 static_assert(sizeof(TileOp) % 8 == 0, "TileOp must stay 8-byte aligned");
 
 constexpr uint32_t NO_PARAM = 0xffffffffu;
+constexpr uint32_t DIAG_RUN_MAX = 32;      // items per OP_DIAG_RUN
+constexpr uint32_t DIAG_RUN_GRADS = 6;     // gradient items per OP_DIAG_RUN
 constexpr int TILE_MAX_BITS = 14;
 
 // Geometry of one fused pass: the sorted tile qubits (global positions).

@@ -686,6 +686,7 @@ void execute_impl(State& s, const std::vector<HOp>& ops, const std::vector<Stage
             for (int oi : gr.ops) micro.push_back(translate_micro<R>(ops[oi], pos, sl, mats));
             items.push_back(it);
         }
+        fuse_diag_runs(items, first[si], false);
         count[si] = items.size() - first[si];
         // fuse the first / last sweep with the HBM load / store on full tiles when
         // its bits leave the 3 lowest tile bits to the lanes (8 consecutive amplitudes)
@@ -744,6 +745,49 @@ void execute_impl(State& s, const std::vector<HOp>& ops, const std::vector<Stage
 }
 
 }  // namespace
+
+// Runs of consecutive diagonal items (>= 2) get an OP_DIAG_RUN header so the
+// kernel applies them in one sweep (limits DIAG_RUN_MAX items, DIAG_RUN_GRADS
+// gradient items per run).  Gradient items join a run only in reverse passes,
+// where phi and lambda receive the same diagonal unitaries.
+void fuse_diag_runs(std::vector<TileOp>& items, size_t first, bool rev) {
+    // Off unless FSA_DIAG_RUNS=1: the one-sweep form did not speed up the
+    // measured workloads and is kept for CPU-side validation only.
+    if (!env_flag("FSA_DIAG_RUNS", false)) return;
+    auto diag = [&](const TileOp& t) {
+        return t.kind == OP_PHASE || (t.kind == OP_PAULI && t.lmask == 0) ||
+               (rev && t.kind == OP_GRAD && t.lmask == 0);
+    };
+    std::vector<TileOp> out(items.begin(), items.begin() + (std::ptrdiff_t)first);
+    size_t i = first;
+    while (i < items.size()) {
+        if (!diag(items[i])) { out.push_back(items[i++]); continue; }
+        size_t j = i;
+        uint32_t ngr = 0;
+        TileOp head;
+        std::memset(&head, 0, sizeof(head));
+        head.kind = OP_DIAG_RUN;
+        head.param = NO_PARAM;
+        while (j < items.size() && diag(items[j]) && j - i < DIAG_RUN_MAX) {
+            if (items[j].kind == OP_GRAD) {
+                if (ngr == DIAG_RUN_GRADS) break;
+                head.p[ngr++] = (uint32_t)(j - i);
+            }
+            j++;
+        }
+        if (j - i >= 2) {
+            head.k = (uint32_t)(j - i);
+            head.lmask2 = ngr;
+            out.push_back(head);
+            out.insert(out.end(), items.begin() + (std::ptrdiff_t)i, items.begin() + (std::ptrdiff_t)j);
+        } else {
+            out.push_back(items[i]);
+            j = i + 1;
+        }
+        i = j;
+    }
+    items.swap(out);
+}
 
 template <typename R>
 TileOp translate_item(const HOp& op, const int* pos, uint64_t tileMask, std::vector<cx<R>>& mats) {

@@ -124,6 +124,37 @@ void flush(State& s);
 template <typename R>
 TileOp translate_item(const HOp& op, const int* pos, uint64_t tileMask, std::vector<cx<R>>& mats);
 
+// Reverse sweep of the adjoint method for an n-qubit state: for k =
+// last..first a gradient item (gadgets with a parameter) then U_k^dag, cut
+// into tile passes over phi and lambda together (or op by op where an op fits
+// no tile).  Pure host code.
+constexpr unsigned REV_MAX_GRADS = 512;     // gradient items per pass (LDS accumulators)
+template <typename R>
+struct RevPlan {
+    struct Pass {
+        bool tiled;
+        BatchStage bs;
+        size_t slot0;                       // first entry of slots for this pass
+        unsigned ngrad;
+        std::vector<int> ops;               // untiled: indices into rev
+    };
+    std::vector<HOp> rev;
+    std::vector<Pass> passes;
+    std::vector<TileOp> items;
+    std::vector<cx<R>> mats;
+    std::vector<uint32_t> slots;            // result slot (1 + parameter) of each gradient item
+    unsigned maxGrad = 0;
+    unsigned m = 0;                         // tile bits
+};
+template <typename R>
+RevPlan<R> build_reverse_plan(const std::vector<HOp>& ops, unsigned n);
+template <typename R>
+int rev_tile_bits();
+
+// give runs of consecutive diagonal items in items[first..] an OP_DIAG_RUN
+// header (rev: gradient items of diagonal strings may join, adjoint passes)
+void fuse_diag_runs(std::vector<TileOp>& items, size_t first, bool rev);
+
 // brackets launches on `stream` with HIP events while profiling is enabled
 class ProfScope {
 public:

@@ -21,6 +21,8 @@
 #include "launch.hpp"
 #include "tile_ops.hpp"
 
+#include <algorithm>
+
 namespace fsa {
 
 namespace {
@@ -114,31 +116,111 @@ __device__ __forceinline__ void apply_terms(const cx<R>* phiAll, cx<R>* lamOut, 
     }
 }
 
+// Gradient items of one stage accumulate in LDS (gacc, indexed by the item's
+// p[0] = its index among the stage's gradient items) over the tiles a
+// workgroup processes, and are written out once per stage (flush_grads).
+constexpr uint32_t MAX_STAGE_GRADS = 512;
+
+// workgroup sums of the first ngr of NG value pairs; thread g < ngr gets sum g
+// in gr[0], gi[0].  red holds 2 * NG * (TB / 64) doubles.
+template <int TB, int NG>
+__device__ __forceinline__ void bsum_multi(double* gr, double* gi, uint32_t ngr, double* red) {
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int g = 0; g < NG; g++) {
+        gr[g] = bwave_sum(gr[g]);
+        gi[g] = bwave_sum(gi[g]);
+    }
+    __syncthreads();                       // red may still be read by a previous sum
+    if (lane == 0) {
+#pragma unroll
+        for (int g = 0; g < NG; g++) {
+            red[2 * (wave * NG + g)] = gr[g];
+            red[2 * (wave * NG + g) + 1] = gi[g];
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < ngr) {
+        double a = 0, b = 0;
+        for (int w = 0; w < TB / 64; w++) {
+            a += red[2 * (w * NG + threadIdx.x)];
+            b += red[2 * (w * NG + threadIdx.x) + 1];
+        }
+        gr[0] = a;
+        gi[0] = b;
+    }
+}
+
 // the items of one stage on the LDS tile(s); OP_GRAD items add their tile sum
-// into res (thread 0, tiles in a fixed order)
+// into gacc (tiles in a fixed order); OP_DIAG_RUN runs take one sweep
 template <typename R, int TB, bool REV>
 __device__ __forceinline__ void run_items(cx<R>* phi, cx<R>* lam, uint32_t T, uint64_t base,
                                           const TileOp* __restrict__ items, uint32_t i0, uint32_t ni,
                                           const cx<R>* __restrict__ mats, const double* __restrict__ params,
-                                          double* __restrict__ res, double* red, bool firstTile) {
+                                          double* gacc, double* red, DiagEnt* dtab, bool firstTile) {
     for (uint32_t i = i0; i < i0 + ni; i++) {
         TileOp op = items[i];
+        if (op.kind == OP_DIAG_RUN) {
+            const uint32_t cnt = op.k, ngr = REV ? op.lmask2 : 0u;
+            if (threadIdx.x < cnt) {
+                TileOp o = items[i + 1 + threadIdx.x];
+                bind_param(o, params);
+                dtab[threadIdx.x] = diag_entry(o, base);
+            }
+            __syncthreads();
+            if (REV && ngr > 0) {
+                double gr[DIAG_RUN_GRADS], gi[DIAG_RUN_GRADS];
+#pragma unroll
+                for (int g = 0; g < (int)DIAG_RUN_GRADS; g++) { gr[g] = 0; gi[g] = 0; }
+                diag_sweep<R, TB, true, DIAG_RUN_GRADS>(phi, lam, T, dtab, cnt, op.p, ngr, gr, gi);
+                bsum_multi<TB, DIAG_RUN_GRADS>(gr, gi, ngr, red);
+                if (threadIdx.x < ngr) {
+                    const DiagEnt& e = dtab[op.p[threadIdx.x]];
+                    const double vr = e.er * gr[0] - e.ei * gi[0];
+                    const double vi = e.er * gi[0] + e.ei * gr[0];
+                    double* r = gacc + 2 * items[i + 1 + op.p[threadIdx.x]].p[0];
+                    if (firstTile) { r[0] = vr; r[1] = vi; }
+                    else { r[0] += vr; r[1] += vi; }
+                }
+            } else {
+                diag_sweep<R, TB, REV, 0>(phi, lam, T, dtab, cnt, nullptr, 0, nullptr, nullptr);
+            }
+            __syncthreads();
+            i += cnt;
+            continue;
+        }
         if (REV && op.kind == OP_GRAD) {
             double vr, vi;
             grad_partial<R, TB>(phi, lam, T, base, op, vr, vi);
             bsum2<TB>(vr, vi, red);
             if (threadIdx.x == 0) {
-                double* r = res + 2 * (size_t)op.mat_off;
+                double* r = gacc + 2 * op.p[0];
                 if (firstTile) { r[0] = vr; r[1] = vi; }
                 else { r[0] += vr; r[1] += vi; }
             }
-            continue;                       // no barrier needed: nothing was written to LDS
+            continue;                       // no barrier needed: nothing was written to the tiles
         }
         bind_param(op, params);
         tile_apply<R, TB, 1>(phi, T, base, op, mats);
         if (REV) tile_apply<R, TB, 1>(lam, T, base, op, mats);
         __syncthreads();
     }
+}
+
+// gradient sums of a finished stage -> res slots (item field mat_off)
+template <int TB>
+__device__ __forceinline__ void flush_grads(const BatchStage& st, const TileOp* __restrict__ items,
+                                            const double* gacc, double* __restrict__ res) {
+    __syncthreads();
+    for (uint32_t i = st.item0 + threadIdx.x; i < st.item0 + st.nitems; i += TB) {
+        if (items[i].kind == OP_GRAD) {
+            const uint32_t g = items[i].p[0];
+            const size_t slot = items[i].mat_off;
+            res[2 * slot] = gacc[2 * g];
+            res[2 * slot + 1] = gacc[2 * g + 1];
+        }
+    }
+    __syncthreads();
 }
 
 template <typename R, int TB, int M>
@@ -152,7 +234,9 @@ __global__ __launch_bounds__(TB) void k_batch(const BatchJob* __restrict__ jobs,
     constexpr uint32_t KMAX = TMAX / TB > 0 ? TMAX / TB : 1;
     __shared__ cx<R> lds[2 * TMAX];
     __shared__ uint64_t offHi[KMAX];
-    __shared__ double red[2 * (TB / 64)];
+    __shared__ double red[2 * DIAG_RUN_GRADS * (TB / 64)];
+    __shared__ double gacc[2 * MAX_STAGE_GRADS];
+    __shared__ DiagEnt dtab[DIAG_RUN_MAX];
     __shared__ unsigned jobIdx;
     const uint32_t tid = threadIdx.x;
 
@@ -172,16 +256,18 @@ __global__ __launch_bounds__(TB) void k_batch(const BatchJob* __restrict__ jobs,
             for (uint32_t l = tid; l < T; l += TB) phi[l] = mk<R>(l == 0 ? (R)1 : (R)0, (R)0);
             __syncthreads();
             for (uint32_t s = J.fwd0; s < J.fwd0 + J.nfwd; s++)
-                run_items<R, TB, false>(phi, lam, T, 0, items, stages[s].item0, stages[s].nitems, mats, params, res,
-                                        red, true);
+                run_items<R, TB, false>(phi, lam, T, 0, items, stages[s].item0, stages[s].nitems, mats, params, gacc,
+                                        red, dtab, true);
             double er = 0, ei = 0;
             apply_terms<R, TB>(phi, lam, 0, T, terms + J.term0, J.nterm, er, ei);
             bsum2<TB>(er, ei, red);
             if (tid == 0) { res[2 * (size_t)J.out] = er; res[2 * (size_t)J.out + 1] = ei; }
             __syncthreads();
-            for (uint32_t s = J.rev0; s < J.rev0 + J.nrev; s++)
-                run_items<R, TB, true>(phi, lam, T, 0, items, stages[s].item0, stages[s].nitems, mats, params, res,
-                                       red, true);
+            for (uint32_t s = J.rev0; s < J.rev0 + J.nrev; s++) {
+                const BatchStage st = stages[s];
+                run_items<R, TB, true>(phi, lam, T, 0, items, st.item0, st.nitems, mats, params, gacc, red, dtab, true);
+                flush_grads<TB>(st, items, gacc, res);
+            }
             continue;
         }
 
@@ -215,10 +301,10 @@ __global__ __launch_bounds__(TB) void k_batch(const BatchJob* __restrict__ jobs,
                     __syncthreads();
                     if (dir == 0)
                         run_items<R, TB, false>(lds, lds + TMAX, T, base, items, st.item0, st.nitems, mats, params,
-                                                res, red, t == 0);
+                                                gacc, red, dtab, t == 0);
                     else
                         run_items<R, TB, true>(lds, lds + TMAX, T, base, items, st.item0, st.nitems, mats, params,
-                                               res, red, t == 0);
+                                               gacc, red, dtab, t == 0);
                     for (uint32_t k = 0; k < K; k++) {
                         const uint32_t l = tid + k * TB;
                         if (l >= T) break;
@@ -228,6 +314,7 @@ __global__ __launch_bounds__(TB) void k_batch(const BatchJob* __restrict__ jobs,
                     }
                     __syncthreads();
                 }
+                if (dir == 1) flush_grads<TB>(st, items, gacc, res);
             }
             if (dir == 0) {
                 // the forward pass is complete in the arena (this workgroup's own
@@ -243,6 +330,73 @@ __global__ __launch_bounds__(TB) void k_batch(const BatchJob* __restrict__ jobs,
                 __syncthreads();
             }
         }
+    }
+}
+
+// One reverse-sweep pass of the single-state adjoint: every workgroup streams
+// its share of the tiles (fixed assignment, fixed order) of phi and lambda
+// through LDS, applies the stage's inverse gates to both, sums each OP_GRAD
+// item over its tiles in LDS and writes the sums to its own row of `part`
+// (item field p[0] = the item's index within the stage).  k_grad_final then sums the rows in
+// workgroup order, so the result does not depend on timing.
+template <typename R, int TB, int M>
+__global__ __launch_bounds__(TB) void k_rev_pass(cx<R>* __restrict__ phi, cx<R>* __restrict__ lam, unsigned n,
+                                                 BatchStage st, const TileOp* __restrict__ items,
+                                                 const cx<R>* __restrict__ mats, double* __restrict__ part,
+                                                 unsigned ngrad) {
+    constexpr uint32_t TMAX = 1u << M;
+    constexpr uint32_t KMAX = TMAX / TB > 0 ? TMAX / TB : 1;
+    __shared__ cx<R> lds[2 * TMAX];
+    __shared__ uint64_t offHi[KMAX];
+    __shared__ double red[2 * DIAG_RUN_GRADS * (TB / 64)];
+    __shared__ double gacc[2 * MAX_STAGE_GRADS];
+    __shared__ DiagEnt dtab[DIAG_RUN_MAX];
+    const uint32_t tid = threadIdx.x;
+    const uint32_t T = 1u << st.m;
+    const uint32_t K = T > TB ? T / TB : 1u;
+    const uint64_t all = (1ULL << n) - 1ULL;
+    const uint64_t outMask = all & ~st.tileMask;
+    const uint64_t numTiles = 1ULL << (n - st.m);
+    if (tid < K) offHi[tid] = pdep64((uint64_t)tid * TB, st.tileMask);
+    const uint64_t offLo = pdep64(tid, st.tileMask);
+    __syncthreads();
+    double* mine = part + (size_t)blockIdx.x * 2 * ngrad;
+    bool first = true;
+    for (uint64_t t = blockIdx.x; t < numTiles; t += gridDim.x) {
+        const uint64_t base = pdep64(t, outMask);
+        for (uint32_t k = 0; k < K; k++) {
+            const uint32_t l = tid + k * TB;
+            if (l >= T) break;
+            const uint64_t g = base | offLo | offHi[k];
+            lds[l] = phi[g];
+            lds[TMAX + l] = lam[g];
+        }
+        __syncthreads();
+        run_items<R, TB, true>(lds, lds + TMAX, T, base, items, 0, st.nitems, mats, nullptr, gacc, red, dtab, first);
+        for (uint32_t k = 0; k < K; k++) {
+            const uint32_t l = tid + k * TB;
+            if (l >= T) break;
+            const uint64_t g = base | offLo | offHi[k];
+            phi[g] = lds[l];
+            lam[g] = lds[TMAX + l];
+        }
+        __syncthreads();
+        first = false;
+    }
+    for (uint32_t i = tid; i < 2 * ngrad; i += TB) mine[i] = gacc[i];
+}
+
+// res[2 slot[i]], res[2 slot[i] + 1] = sum over the rows of part of item i's sums
+__global__ void k_grad_final(const double* __restrict__ part, unsigned rows, unsigned ngrad,
+                             const uint32_t* __restrict__ slots, double* __restrict__ res) {
+    for (unsigned i = threadIdx.x; i < ngrad; i += blockDim.x) {
+        double a = 0, b = 0;
+        for (unsigned w = 0; w < rows; w++) {
+            a += part[(size_t)w * 2 * ngrad + 2 * i];
+            b += part[(size_t)w * 2 * ngrad + 2 * i + 1];
+        }
+        res[2 * (size_t)slots[i]] = a;
+        res[2 * (size_t)slots[i] + 1] = b;
     }
 }
 
@@ -289,6 +443,40 @@ bool launch_batch(hipStream_t s, int tb, int m, unsigned grid, const BatchJob* j
     return false;
 }
 
+template <typename R>
+int rev_tile_bits() { return sizeof(R) == 8 ? 11 : 12; }
+
+// reverse-pass workgroups: one per tile up to what the device holds at once
+template <typename R>
+unsigned rev_grid(uint64_t numTiles) {
+    constexpr int M = sizeof(R) == 8 ? 11 : 12;
+    int dev = 0, cus = 0, blocks = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, reinterpret_cast<const void*>(&k_rev_pass<R, 512, M>),
+                                                     512, 0) != hipSuccess || blocks < 1)
+        return 0;
+    return (unsigned)std::min<uint64_t>(numTiles, (uint64_t)cus * blocks);
+}
+
+template <typename R>
+void launch_rev_pass(hipStream_t s, unsigned grid, cx<R>* phi, cx<R>* lam, unsigned n, const BatchStage& st,
+                     const TileOp* items, const cx<R>* mats, double* part, unsigned ngrad, const uint32_t* slots,
+                     double* res) {
+    constexpr int M = sizeof(R) == 8 ? 11 : 12;
+    hipLaunchKernelGGL((k_rev_pass<R, 512, M>), dim3(grid), dim3(512), 0, s, phi, lam, n, st, items, mats, part,
+                       ngrad);
+    if (ngrad) hipLaunchKernelGGL(k_grad_final, dim3(1), dim3(256), 0, s, part, grid, ngrad, slots, res);
+}
+
+template int rev_tile_bits<double>();
+template int rev_tile_bits<float>();
+template unsigned rev_grid<double>(uint64_t);
+template unsigned rev_grid<float>(uint64_t);
+template void launch_rev_pass<double>(hipStream_t, unsigned, cx<double>*, cx<double>*, unsigned, const BatchStage&,
+                                      const TileOp*, const cx<double>*, double*, unsigned, const uint32_t*, double*);
+template void launch_rev_pass<float>(hipStream_t, unsigned, cx<float>*, cx<float>*, unsigned, const BatchStage&,
+                                     const TileOp*, const cx<float>*, double*, unsigned, const uint32_t*, double*);
 template int batch_max_bits<double>();
 template int batch_max_bits<float>();
 template int batch_occupancy<double>(int, int);

@@ -309,6 +309,7 @@ __global__ __launch_bounds__(TB, 4) void k_tile(cx<R>* __restrict__ a, TileGeom 
                                                 const TileOp* __restrict__ micro, const cx<R>* __restrict__ mats,
                                                 uint64_t numTiles, uint32_t flags) {
     extern __shared__ unsigned char smem_raw[];
+    __shared__ DiagEnt dtab[DIAG_RUN_MAX];
     cx<R>* s = reinterpret_cast<cx<R>*>(smem_raw);
     constexpr int LOGTB = TB == 1024 ? 10 : TB == 512 ? 9 : TB == 256 ? 8 : 7;
     constexpr int LOGROWS = ROWS >= 16 ? 4 : ROWS >= 8 ? 3 : ROWS >= 4 ? 2 : ROWS >= 2 ? 1 : 0;
@@ -377,6 +378,15 @@ __global__ __launch_bounds__(TB, 4) void k_tile(cx<R>* __restrict__ a, TileGeom 
         }
         for (unsigned it = 0; it < numItems; it++) {
             const TileOp& op = items[it];
+            if (op.kind == OP_DIAG_RUN) {
+                const uint32_t cnt = op.k;
+                if (tid < cnt) dtab[tid] = diag_entry(items[it + 1 + tid], base);
+                __syncthreads();
+                diag_sweep<R, TB, false, 0>(s, nullptr, T, dtab, cnt, nullptr, 0, nullptr, nullptr);
+                __syncthreads();
+                it += cnt;
+                continue;
+            }
             if (!SW || op.kind != OP_SWEEP) {
                 tile_apply<R, TB, VAR>(s, T, base, op, mats);
                 __syncthreads();

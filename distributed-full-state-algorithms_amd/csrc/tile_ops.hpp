@@ -366,5 +366,83 @@ __device__ __forceinline__ void tile_apply(cx<R>* s, uint32_t T, uint64_t base, 
 }
 
 
+// ---- runs of diagonal ops -----------------------------------------------------
+// An OP_DIAG_RUN header (k = number of items that follow it) marks k
+// consecutive items that are all diagonal in the tile: OP_PHASE, OP_PAULI
+// with no X/Y qubit in the tile (lmask == 0) and, in adjoint reverse passes,
+// OP_GRAD of a diagonal Pauli string.  They are applied in ONE sweep over the
+// tile instead of one sweep (and one barrier) each: every amplitude is read
+// once, multiplied by each op's parity-selected factor, and written once.
+// Gradient items of the run (header p[0..lmask2-1] = their positions in the
+// run) are all evaluated on the run's input state: a diagonal P commutes with
+// the run's diagonal unitaries, and those act on phi and lambda alike, so
+// <D lambda|P|D phi> = <lambda|P|phi> for every such D.
+
+struct DiagEnt {
+    double er, ei;        // factor on even parity (gradient items: i^#Y)
+    double orr, oi;       // factor on odd parity
+    uint32_t mask;        // tile-local parity mask
+    uint32_t bp;          // parity of the tile's base index under the out-of-tile mask
+    uint32_t grad;        // 1 for a gradient item (no factor applied)
+    uint32_t pad;
+};
+
+// item -> table entry for the tile with base index `base` (gadget factors already bound)
+__device__ __forceinline__ DiagEnt diag_entry(const TileOp& o, uint64_t base) {
+    DiagEnt e;
+    e.bp = par64(base & o.gmask);
+    e.grad = 0;
+    e.pad = 0;
+    if (o.kind == OP_PHASE) {
+        e.mask = o.lmask;
+        e.er = o.f[0]; e.ei = o.f[1];
+        e.orr = o.f[2]; e.oi = o.f[3];
+    } else if (o.kind == OP_PAULI) {          // th * v + (+-fe) * v
+        e.mask = o.lmask2;
+        e.er = o.f[0] + o.f[2]; e.ei = o.f[1] + o.f[3];
+        e.orr = o.f[0] - o.f[2]; e.oi = o.f[1] - o.f[3];
+    } else {                                  // OP_GRAD
+        e.mask = o.lmask2;
+        e.er = o.f[0]; e.ei = o.f[1];
+        e.orr = 0; e.oi = 0;
+        e.grad = 1;
+    }
+    return e;
+}
+
+// One sweep over the tile(s) with the run's table in LDS.  TWO: apply to s2
+// as well (phi and lambda).  NG > 0: accumulate sum_l sign_g(l) conj(s2_l) s_l
+// for the run's first NG gradient entries (positions gidx) into gr/gi.
+template <typename R, int TB, bool TWO, int NG>
+__device__ __forceinline__ void diag_sweep(cx<R>* s, cx<R>* s2, uint32_t T, const DiagEnt* tab, uint32_t count,
+                                           const uint32_t* gidx, uint32_t ngr, double* gr, double* gi) {
+    for (uint32_t l = threadIdx.x; l < T; l += TB) {
+        cx<R> v = s[l];
+        cx<R> w = TWO ? s2[l] : v;
+        if constexpr (NG > 0) {
+            const double xr = (double)w.x * (double)v.x + (double)w.y * (double)v.y;
+            const double xi = (double)w.x * (double)v.y - (double)w.y * (double)v.x;
+#pragma unroll
+            for (int g = 0; g < NG; g++) {
+                if ((uint32_t)g >= ngr) break;
+                const DiagEnt& e = tab[gidx[g]];
+                const double sg = (e.bp ^ par32(l & e.mask)) ? -1.0 : 1.0;
+                gr[g] += sg * xr;
+                gi[g] += sg * xi;
+            }
+        }
+        for (uint32_t j = 0; j < count; j++) {
+            const DiagEnt& e = tab[j];
+            if (e.grad) continue;
+            const bool odd = (e.bp ^ par32(l & e.mask)) != 0;
+            const cx<R> f = odd ? mk<R>((R)e.orr, (R)e.oi) : mk<R>((R)e.er, (R)e.ei);
+            v = cmul(v, f);
+            if (TWO) w = cmul(w, f);
+        }
+        s[l] = v;
+        if (TWO) s2[l] = w;
+    }
+}
+
 }  // namespace tile
 }  // namespace fsa
