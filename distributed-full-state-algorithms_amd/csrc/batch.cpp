@@ -150,7 +150,6 @@ struct BlobBuilder {
                 TileOp t = translate_item<R>(op, pos, bs.tileMask, mats);
                 if (op.kind == OP_GRAD && ngrad == MAX_STAGE_GRADS_HOST) {
                     // close the pass and continue the same ops in a new one on the same tiles
-                    fuse_diag_runs(items, bs.item0, rev);
                     bs.nitems = (uint32_t)(items.size() - bs.item0);
                     stages.push_back(bs);
                     bs.item0 = (uint32_t)items.size();
@@ -170,7 +169,6 @@ struct BlobBuilder {
                 }
                 items.push_back(t);
             }
-            fuse_diag_runs(items, bs.item0, rev);
             bs.nitems = (uint32_t)(items.size() - bs.item0);
             stages.push_back(bs);
         }

@@ -153,14 +153,16 @@ __device__ __forceinline__ void bsum_multi(double* gr, double* gi, uint32_t ngr,
 
 // the items of one stage on the LDS tile(s); OP_GRAD items add their tile sum
 // into gacc (tiles in a fixed order); OP_DIAG_RUN runs take one sweep
-template <typename R, int TB, bool REV>
+// RUNS: the plan may hold OP_DIAG_RUN headers (the batch kernel's plans never
+// do, so its instantiation leaves that code out)
+template <typename R, int TB, bool REV, bool RUNS>
 __device__ __forceinline__ void run_items(cx<R>* phi, cx<R>* lam, uint32_t T, uint64_t base,
                                           const TileOp* __restrict__ items, uint32_t i0, uint32_t ni,
                                           const cx<R>* __restrict__ mats, const double* __restrict__ params,
                                           double* gacc, double* red, DiagEnt* dtab, bool firstTile) {
     for (uint32_t i = i0; i < i0 + ni; i++) {
         TileOp op = items[i];
-        if (op.kind == OP_DIAG_RUN) {
+        if (RUNS && op.kind == OP_DIAG_RUN) {
             const uint32_t cnt = op.k, ngr = REV ? op.lmask2 : 0u;
             if (threadIdx.x < cnt) {
                 TileOp o = items[i + 1 + threadIdx.x];
@@ -256,7 +258,7 @@ __global__ __launch_bounds__(TB) void k_batch(const BatchJob* __restrict__ jobs,
             for (uint32_t l = tid; l < T; l += TB) phi[l] = mk<R>(l == 0 ? (R)1 : (R)0, (R)0);
             __syncthreads();
             for (uint32_t s = J.fwd0; s < J.fwd0 + J.nfwd; s++)
-                run_items<R, TB, false>(phi, lam, T, 0, items, stages[s].item0, stages[s].nitems, mats, params, gacc,
+                run_items<R, TB, false, false>(phi, lam, T, 0, items, stages[s].item0, stages[s].nitems, mats, params, gacc,
                                         red, dtab, true);
             double er = 0, ei = 0;
             apply_terms<R, TB>(phi, lam, 0, T, terms + J.term0, J.nterm, er, ei);
@@ -265,7 +267,7 @@ __global__ __launch_bounds__(TB) void k_batch(const BatchJob* __restrict__ jobs,
             __syncthreads();
             for (uint32_t s = J.rev0; s < J.rev0 + J.nrev; s++) {
                 const BatchStage st = stages[s];
-                run_items<R, TB, true>(phi, lam, T, 0, items, st.item0, st.nitems, mats, params, gacc, red, dtab, true);
+                run_items<R, TB, true, false>(phi, lam, T, 0, items, st.item0, st.nitems, mats, params, gacc, red, dtab, true);
                 flush_grads<TB>(st, items, gacc, res);
             }
             continue;
@@ -300,10 +302,10 @@ __global__ __launch_bounds__(TB) void k_batch(const BatchJob* __restrict__ jobs,
                     }
                     __syncthreads();
                     if (dir == 0)
-                        run_items<R, TB, false>(lds, lds + TMAX, T, base, items, st.item0, st.nitems, mats, params,
+                        run_items<R, TB, false, false>(lds, lds + TMAX, T, base, items, st.item0, st.nitems, mats, params,
                                                 gacc, red, dtab, t == 0);
                     else
-                        run_items<R, TB, true>(lds, lds + TMAX, T, base, items, st.item0, st.nitems, mats, params,
+                        run_items<R, TB, true, false>(lds, lds + TMAX, T, base, items, st.item0, st.nitems, mats, params,
                                                gacc, red, dtab, t == 0);
                     for (uint32_t k = 0; k < K; k++) {
                         const uint32_t l = tid + k * TB;
@@ -372,7 +374,7 @@ __global__ __launch_bounds__(TB) void k_rev_pass(cx<R>* __restrict__ phi, cx<R>*
             lds[TMAX + l] = lam[g];
         }
         __syncthreads();
-        run_items<R, TB, true>(lds, lds + TMAX, T, base, items, 0, st.nitems, mats, nullptr, gacc, red, dtab, first);
+        run_items<R, TB, true, true>(lds, lds + TMAX, T, base, items, 0, st.nitems, mats, nullptr, gacc, red, dtab, first);
         for (uint32_t k = 0; k < K; k++) {
             const uint32_t l = tid + k * TB;
             if (l >= T) break;
