@@ -69,8 +69,9 @@ static bool env_flag(const char* name, bool dflt) {
 }
 
 SchedParams sched_params(unsigned nq, int precision) {
-    unsigned mmax = precision == FSA_FP64 ? (unsigned)tile_bits_max<double>() : (unsigned)tile_bits_max<float>();
-    unsigned m = std::min(env_u("FSA_TILE_BITS", mmax), mmax);
+    const unsigned mmax = precision == FSA_FP64 ? (unsigned)tile_bits_max<double>() : (unsigned)tile_bits_max<float>();
+    const unsigned mlim = precision == FSA_FP64 ? (unsigned)tile_bits_limit<double>() : (unsigned)tile_bits_limit<float>();
+    unsigned m = std::min(env_u("FSA_TILE_BITS", mmax), mlim);
     // Qubits 0..low-1 are always tile qubits, so every HBM access of a pass is
     // a run of at least 2^low consecutive amplitudes: 128 bytes (one line) in
     // either precision.  Fewer forced qubits leave more tile slots for gates
@@ -648,7 +649,7 @@ void execute_impl(State& s, const std::vector<HOp>& ops, const std::vector<Stage
         // the MFMA kernel variant carries no VALU dense gates: a pass uses it only
         // when every dense gate in it has the MFMA form (k = 3..5 on a full tile)
         const bool useMfma = env_flag("FSA_MFMA", true);
-        bool stageMfma = useMfma && m == FULL;
+        bool stageMfma = useMfma && m >= FULL;        // full tiles of 2^12 (FP64) / 2^13 amplitudes or more
         bool anyMfma = false;
         for (int oi : st.ops) {
             const HOp& op = ops[oi];

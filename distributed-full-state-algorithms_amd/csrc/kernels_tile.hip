@@ -8,6 +8,7 @@
 // coalesced write).  Controls and Z-type factors on qubits outside the tile
 // are constant over a workgroup and are folded into a per-tile test or sign.
 // The arithmetic per amplitude is the same as the streaming kernels'.
+#include "engine.hpp"
 #include "launch.hpp"
 #include "tile_ops.hpp"
 
@@ -304,7 +305,7 @@ __device__ __forceinline__ uint64_t tile_scatter(uint32_t l, const TileGeom& g) 
 // last item is a sweep fused with the HBM store.  A pass made of a single
 // fused sweep never touches LDS.  ROWS = tile size / TB for row-layout loads.
 template <typename R, int TB, int ROWS, int W, int VAR, bool SW>
-__global__ __launch_bounds__(TB, 4) void k_tile(cx<R>* __restrict__ a, TileGeom geom,
+__global__ __launch_bounds__(TB, (sizeof(R) == 8 && ROWS >= 16) ? 2 : 4) void k_tile(cx<R>* __restrict__ a, TileGeom geom,
                                                 const TileOp* __restrict__ items, unsigned numItems,
                                                 const TileOp* __restrict__ micro, const cx<R>* __restrict__ mats,
                                                 uint64_t numTiles, uint32_t flags) {
@@ -388,7 +389,7 @@ __global__ __launch_bounds__(TB, 4) void k_tile(cx<R>* __restrict__ a, TileGeom 
                 continue;
             }
             if (!SW || op.kind != OP_SWEEP) {
-                tile_apply<R, TB, VAR>(s, T, base, op, mats);
+                tile_apply<R, TB, VAR, (ROWS * TB >= 8192 ? ROWS * TB / 4096 : 1)>(s, T, base, op, mats);
                 __syncthreads();
                 continue;
             }
@@ -454,8 +455,9 @@ void launch_tile(hipStream_t s, cx<R>* a, unsigned nq, const TileGeom& geom, con
     }
     uint64_t numTiles = 1ULL << (nq - geom.m);
     size_t lds = sizeof(cx<R>) << geom.m;
-    // enough resident workgroups to fill every CU (LDS allows two 64 KiB tiles per CU)
-    uint64_t grid = std::min<uint64_t>(numTiles, (uint64_t)numCU * 2);
+    // enough resident workgroups to fill every CU: two 64 KiB tiles per CU, one 128 KiB tile
+    const uint64_t perCU = lds > 80 * 1024 ? 1 : 2;
+    uint64_t grid = std::min<uint64_t>(numTiles, (uint64_t)numCU * perCU);
     const uint32_t rows = geom.m > 9 ? 1u << (geom.m - 9) : 1u;
     const uint32_t var = (flags >> 2) & 3u;
     const bool sw = (flags & 16u) != 0;
@@ -468,9 +470,15 @@ void launch_tile(hipStream_t s, cx<R>* a, unsigned nq, const TileGeom& geom, con
     case 2: FSA_PICK(2) break;
     case 4: FSA_PICK(4) break;
     case 8: FSA_PICK(8) break;
-    default: FSA_PICK((sizeof(R) == 8 ? 8 : 16)) break;
+    default: FSA_PICK(16) break;
     }
 #undef FSA_PICK
+    if (lds > 64 * 1024) {
+        // dynamic LDS beyond 64 KiB must be granted per kernel before the launch
+        if (hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)lds) != hipSuccess)
+            fail(FSA_ERR_HIP, "cannot grant 128 KiB of LDS to the tile kernel");
+    }
     hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(TB), lds, s, a, geom, ditems, numItems, dmicro, dmats,
                        numTiles, flags);
 }

@@ -46,6 +46,8 @@ void launch_tile(hipStream_t s, cx<R>* a, unsigned nq, const TileGeom& geom, con
                  unsigned numItems, const TileOp* dmicro, const cx<R>* dmats, uint32_t flags);
 // largest tile (log2 amplitudes) the LDS budget allows for this precision
 template <typename R> constexpr int tile_bits_max() { return sizeof(R) == 8 ? 12 : 13; }
+// largest tile a pass may use when asked for (FSA_TILE_BITS): 128 KiB of LDS
+template <typename R> constexpr int tile_bits_limit() { return sizeof(R) == 8 ? 13 : 13; }
 // register-sweep width: 2^W amplitudes per thread = full tile / 512 threads
 template <typename R> constexpr int sweep_bits() { return sizeof(R) == 8 ? 3 : 4; }
 

@@ -75,45 +75,48 @@ __device__ __forceinline__ void tile_matk(cx<R>* s, uint32_t T, const TileOp& op
         }
         return;
     }
-    // split mode (one group per thread, tpg threads per group, outs consecutive
-    // outputs each): all inputs are read before the barrier, outputs written after
-    const uint32_t byThreads = TB / groups;
-    const uint32_t tpg = byThreads < (uint32_t)(D / OUT) ? byThreads : (uint32_t)(D / OUT);
-    const uint32_t outs = D / tpg;
-    const uint32_t g = tid & (groups - 1), sub = tid / groups;
-    const bool active = sub < tpg;
-    const uint32_t r0 = sub * outs;
-    const uint32_t b = ins0_mask32(g, tm);
-    cx<R> acc[OUT];
+    // split mode: tpg threads per group, each computing outs = D / tpg <= OUT
+    // consecutive outputs; a round covers TB / tpg whole groups, all of whose
+    // inputs are read before the barrier and outputs written after it (rounds
+    // touch disjoint groups, so they need no barrier between them)
+    constexpr uint32_t TPG = D > OUT ? (uint32_t)(D / OUT) : 1u;
+    constexpr uint32_t OUTS = (uint32_t)D / TPG;
+    constexpr uint32_t GPR = (uint32_t)TB / TPG;             // groups per round
+    const uint32_t gl = tid % GPR, sub = tid / GPR;
+    const uint32_t r0 = sub * OUTS;
+    for (uint32_t g0 = 0; g0 < groups; g0 += GPR) {
+        const uint32_t g = g0 + gl;
+        const bool active = g < groups;
+        const uint32_t b = ins0_mask32(active ? g : 0u, tm);
+        cx<R> acc[OUT];
 #pragma unroll
-    for (int o = 0; o < OUT; o++) acc[o] = mk<R>(0, 0);
-    if (groups >= 64 && outs == (uint32_t)OUT) {
-        // a wave spans one `sub`, so its output rows are wave-uniform: read the
-        // matrix through the scalar unit instead of one vector load per lane
-        const uint32_t r0u = __builtin_amdgcn_readfirstlane(r0);
-        const cx<R>* mrow = mat + (size_t)r0u * D;
-        if (active) {
+        for (int o = 0; o < OUT; o++) acc[o] = mk<R>(0, 0);
+        if (GPR >= 64) {
+            // a wave spans one `sub`, so its output rows are wave-uniform: read the
+            // matrix through the scalar unit instead of one vector load per lane
+            const uint32_t r0u = __builtin_amdgcn_readfirstlane(r0);
+            const cx<R>* mrow = mat + (size_t)r0u * D;
+            if (active) {
+#pragma unroll 1
+                for (int l = 0; l < D; l++) {
+                    cx<R> x = s[b | FSA_OFF((uint32_t)l)];
+#pragma unroll
+                    for (int o = 0; o < (int)OUTS; o++) acc[o] = cfma(acc[o], mrow[o * D + l], x);
+                }
+            }
+        } else if (active) {
 #pragma unroll 1
             for (int l = 0; l < D; l++) {
                 cx<R> x = s[b | FSA_OFF((uint32_t)l)];
 #pragma unroll
-                for (int o = 0; o < OUT; o++) acc[o] = cfma(acc[o], mrow[o * D + l], x);
+                for (int o = 0; o < (int)OUTS; o++) acc[o] = cfma(acc[o], mat[(r0 + o) * D + l], x);
             }
         }
-    } else if (active) {
-#pragma unroll 1
-        for (int l = 0; l < D; l++) {
-            cx<R> x = s[b | FSA_OFF((uint32_t)l)];
+        __syncthreads();
+        if (active) {
 #pragma unroll
-            for (int o = 0; o < OUT; o++)
-                if ((uint32_t)o < outs) acc[o] = cfma(acc[o], mat[(r0 + o) * D + l], x);
+            for (int o = 0; o < (int)OUTS; o++) s[b | FSA_OFF(r0 + o)] = acc[o];
         }
-    }
-    __syncthreads();
-    if (active) {
-#pragma unroll
-        for (int o = 0; o < OUT; o++)
-            if ((uint32_t)o < outs) s[b | FSA_OFF(r0 + o)] = acc[o];
     }
 #undef FSA_OFF
 }
@@ -144,14 +147,15 @@ template <> struct MfmaAcc<float> {
     }
 };
 
-// tasks per wave for a full tile: FP64 (4096 amplitudes) 1, FP32 (8192) 2
-template <typename R, int K, int TB>
+// A tile of T amplitudes holds T / 512 wave tasks for every K (RB * groups / 64),
+// so each of the TB / 64 waves takes MAXT = T / (8 TB) of them: 1 for 4096
+// amplitudes at TB = 512, 2 for 8192.
+template <typename R, int K, int TB, int MAXT>
 __device__ __forceinline__ void tile_matk_mfma(cx<R>* s, uint32_t T, const TileOp& op, const R* __restrict__ M) {
     typedef typename MfmaAcc<R>::T acc_t;
     constexpr int D = 1 << K, D2 = 2 * D;
     constexpr int RB = D2 / 16;                 // 16-row blocks of the real matrix
     constexpr int WAVES = TB / 64;
-    constexpr int MAXT = sizeof(R) == 8 ? 1 : 2;
     const uint32_t b0 = 1u << op.p[0], b1 = 1u << op.p[1], b2 = 1u << op.p[2];
     const uint32_t b3 = K > 3 ? 1u << op.p[3] : 0u;
     const uint32_t b4 = K > 4 ? 1u << op.p[4] : 0u;
@@ -211,7 +215,7 @@ __device__ __forceinline__ void tile_matk_mfma(cx<R>* s, uint32_t T, const TileO
 // sweeps); VAR 1 adds VALU dense gates and two-qubit depolarising; VAR 2 has
 // MFMA dense gates instead of VALU ones.  Each instantiation stays within 128
 // VGPRs so two 64 KiB workgroups share a CU.
-template <typename R, int TB, int VAR>
+template <typename R, int TB, int VAR, int MAXT = 1>
 __device__ __forceinline__ void tile_apply(cx<R>* s, uint32_t T, uint64_t base, const TileOp& op,
                            const cx<R>* __restrict__ mats) {
     const uint32_t tid = threadIdx.x;
@@ -277,9 +281,9 @@ __device__ __forceinline__ void tile_apply(cx<R>* s, uint32_t T, uint64_t base, 
         if (op.lmask2 == 1u) {        // host chose the MFMA form (full tile, k = 3, 4 or 5)
             if constexpr (VAR == 2) {
                 const R* M = reinterpret_cast<const R*>(mat);
-                if (op.k == 3) tile_matk_mfma<R, 3, TB>(s, T, op, M);
-                else if (op.k == 4) tile_matk_mfma<R, 4, TB>(s, T, op, M);
-                else tile_matk_mfma<R, 5, TB>(s, T, op, M);
+                if (op.k == 3) tile_matk_mfma<R, 3, TB, MAXT>(s, T, op, M);
+                else if (op.k == 4) tile_matk_mfma<R, 4, TB, MAXT>(s, T, op, M);
+                else tile_matk_mfma<R, 5, TB, MAXT>(s, T, op, M);
             }
             return;
         }

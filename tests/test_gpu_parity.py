@@ -84,7 +84,8 @@ def test_sv_single_ops(fsa, orc, mode, n):
 
 
 TUNINGS = {"default": {}, "valu-dense": {"FSA_MFMA": "0"}, "sweeps": {"FSA_SWEEPS": "1"},
-           "sweeps-valu": {"FSA_SWEEPS": "1", "FSA_MFMA": "0"}}
+           "sweeps-valu": {"FSA_SWEEPS": "1", "FSA_MFMA": "0"}, "tile13": {"FSA_TILE_BITS": "13"},
+           "tile13-low5": {"FSA_TILE_BITS": "13", "FSA_TILE_LOW": "5"}}
 
 
 @pytest.mark.parametrize("n", [4, 9, 12, 14, 18, 20])
@@ -104,6 +105,28 @@ def test_sv_fused_circuits(fsa, orc, n, tuning, monkeypatch):
         name, args = random_sv_op(rng, n)
         apply_sv(fsa, orc, psi, ref, name, args)
     assert rel(psi.download(), ref) <= TOL64
+
+
+@pytest.mark.parametrize("n", [14, 17])
+@pytest.mark.parametrize("tuning", ["default", "valu-dense", "tile13"])
+def test_dense_only_circuits(fsa, orc, n, tuning, monkeypatch):
+    """Passes holding only 3-5 target dense gates (the MFMA form on full tiles of
+    2^12 and 2^13 amplitudes, the VALU form otherwise), unitary matrices."""
+    for k, v in TUNINGS[tuning].items():
+        monkeypatch.setenv(k, v)
+    rng = np.random.default_rng(900 + n)
+    for precision in ("fp64", "fp32"):
+        psi = fsa.StateVector(n, precision=precision, mode=1)
+        v = rs(rng, n)
+        psi.upload(v)
+        ref = v.copy()
+        for _ in range(30):
+            k = int(rng.integers(3, 6))
+            q = [int(x) for x in rng.permutation(n)[:k]]
+            u, _, vh = np.linalg.svd(rm(rng, 1 << k))
+            apply_sv(fsa, orc, psi, ref, "manyTargGate", (q, u @ vh))
+        tol = TOL64 if precision == "fp64" else TOL32 * np.sqrt(30)
+        assert rel(psi.download(), ref) <= tol, (precision, tuning)
 
 
 def test_sv_gadget_limits(fsa, orc):
