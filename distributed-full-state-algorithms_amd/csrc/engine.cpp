@@ -369,7 +369,7 @@ TileOp translate(const HOp& op, const int* pos, uint64_t tileMask, std::vector<c
         t.k = k;
         for (size_t i = 0; i < op.q.size(); i++) t.p[i] = (uint32_t)pos[op.q[i]];
         t.mat_off = (uint32_t)mats.size();
-        if (mfma) {
+        if (mfma && k >= 3) {
             // MFMA form: real 2D x 2D matrix [[Ur, -Ui], [Ui, Ur]], row-major, in
             // complex slots of the parameter buffer (two reals per slot)
             t.lmask2 = 1;
@@ -655,7 +655,7 @@ void execute_impl(State& s, const std::vector<HOp>& ops, const std::vector<Stage
             const HOp& op = ops[oi];
             if (op.kind != OP_MATK) continue;
             if (op.q.size() >= 3 && op.q.size() <= 5) anyMfma = true;
-            else stageMfma = false;
+            else if (op.q.size() != 2) stageMfma = false;   // two-target gates run on VALU in either variant
         }
         stageMfma = stageMfma && anyMfma;
         for (int oi : st.ops) {
@@ -713,7 +713,7 @@ void execute_impl(State& s, const std::vector<HOp>& ops, const std::vector<Stage
             if (items[i].kind == OP_SWEEP) hasSweep = true;
         }
         for (size_t i = first[si]; i < first[si] + count[si]; i++)
-            if (items[i].kind == OP_MATK && (items[i].lmask2 == 1u) != (var == 2u))
+            if (items[i].kind == OP_MATK && (items[i].lmask2 == 1u) != (var == 2u) && !(var == 2u && items[i].k == 2))
                 fail(FSA_ERR_INTERNAL, "tile pass mixes MFMA and VALU dense gates");
         flags[si] |= var << 2;
         if (hasSweep) flags[si] |= 16u;

@@ -287,7 +287,11 @@ __device__ __forceinline__ void tile_apply(cx<R>* s, uint32_t T, uint64_t base, 
             }
             return;
         }
-        if constexpr (VAR == 2) return;
+        if constexpr (VAR == 2) {
+            // the MFMA variant also carries two-target gates (VALU, in place)
+            if (op.k == 2) tile_matk<R, 2, TB>(s, T, op, mat);
+            return;
+        }
         switch (op.k) {
         case 1: tile_matk<R, 1, TB>(s, T, op, mat); break;
         case 2: tile_matk<R, 2, TB>(s, T, op, mat); break;
