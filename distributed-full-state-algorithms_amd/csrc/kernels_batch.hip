@@ -291,16 +291,34 @@ __global__ __launch_bounds__(TB) void k_batch(const BatchJob* __restrict__ jobs,
                 const uint64_t offLo = pdep64(tid, st.tileMask);
                 __syncthreads();
                 const bool fromZero = dir == 0 && s == s0;   // the first pass starts from |0..0>
+                // Tiles of one pass are disjoint, so tile t + 1 is loaded into
+                // registers while tile t is worked on in LDS.  A tile here has
+                // T >= 2^10 >= TB amplitudes, K = T / TB <= KMAX per thread.
+                R px[KMAX], py[KMAX], qx[KMAX], qy[KMAX];
+#define FSA_BATCH_LOAD(tt)                                                                          \
+    do {                                                                                            \
+        const uint64_t b_ = pdep64((tt), outMask);                                                  \
+        _Pragma("unroll") for (uint32_t k = 0; k < KMAX; k++) {                                     \
+            if (k < K) {                                                                            \
+                const uint64_t g = b_ | offLo | offHi[k];                                           \
+                const cx<R> v = fromZero ? mk<R>(g == 0 ? (R)1 : (R)0, (R)0) : gphi[g];            \
+                px[k] = v.x; py[k] = v.y;                                                           \
+                if (dir == 1) { const cx<R> w = glam[g]; qx[k] = w.x; qy[k] = w.y; }               \
+            }                                                                                       \
+        }                                                                                           \
+    } while (0)
+                FSA_BATCH_LOAD(0);
                 for (uint64_t t = 0; t < numTiles; t++) {
                     const uint64_t base = pdep64(t, outMask);
-                    for (uint32_t k = 0; k < K; k++) {
-                        const uint32_t l = tid + k * TB;
-                        if (l >= T) break;
-                        const uint64_t g = base | offLo | offHi[k];
-                        lds[l] = fromZero ? mk<R>(g == 0 ? (R)1 : (R)0, (R)0) : gphi[g];
-                        if (dir == 1) lds[TMAX + l] = glam[g];
+#pragma unroll
+                    for (uint32_t k = 0; k < KMAX; k++) {
+                        if (k < K) {
+                            lds[tid + k * TB] = mk<R>(px[k], py[k]);
+                            if (dir == 1) lds[TMAX + tid + k * TB] = mk<R>(qx[k], qy[k]);
+                        }
                     }
                     __syncthreads();
+                    if (t + 1 < numTiles) FSA_BATCH_LOAD(t + 1);
                     if (dir == 0)
                         run_items<R, TB, false, false>(lds, lds + TMAX, T, base, items, st.item0, st.nitems, mats, params,
                                                 gacc, red, dtab, t == 0);
@@ -309,13 +327,13 @@ __global__ __launch_bounds__(TB) void k_batch(const BatchJob* __restrict__ jobs,
                                                gacc, red, dtab, t == 0);
                     for (uint32_t k = 0; k < K; k++) {
                         const uint32_t l = tid + k * TB;
-                        if (l >= T) break;
                         const uint64_t g = base | offLo | offHi[k];
                         gphi[g] = lds[l];
                         if (dir == 1) glam[g] = lds[TMAX + l];
                     }
                     __syncthreads();
                 }
+#undef FSA_BATCH_LOAD
                 if (dir == 1) flush_grads<TB>(st, items, gacc, res);
             }
             if (dir == 0) {
