@@ -1,0 +1,221 @@
+"""libfsa's kernel source run on the CPU under AddressSanitizer.
+
+tests/emu/ holds a CPU stand-in for the HIP runtime and device built-ins (one
+OS thread per lane, real barriers, wave exchanges, the 16x16x4 MFMA with the
+operand layout measured on gfx950).  The real planner (engine.cpp), the fused
+tile kernel (kernels_tile.hip) and the batch kernel (kernels_batch.hip) are
+compiled against it with clang++ -fsanitize=address, so every LDS and global
+access of those kernels is bounds-checked on the host, and their results are
+compared with the oracle:
+
+  * fused passes of 3-5 target dense gates on 2^12- and 2^13-amplitude tiles
+    (MFMA and VALU forms; the VALU split mode for 2^13 FP64 tiles was wrong
+    before it was restructured into rounds of whole groups);
+  * a mixed random circuit on 2^13-amplitude tiles;
+  * batched <H> and adjoint gradients, LDS-resident and streamed jobs, FP64 and
+    FP32, against an adjoint computed with the oracle.
+
+Skipped when the ROCm clang++ is not installed.
+"""
+import os
+import shutil
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+ROOT = Path(__file__).resolve().parent.parent
+EMU = ROOT / "tests" / "emu"
+CSRC = ROOT / "distributed-full-state-algorithms_amd" / "csrc"
+BUILD = EMU / "_build"
+CXX = os.environ.get("FSA_EMU_CXX", "/opt/rocm/llvm/bin/clang++")
+FLAGS = ["-std=c++20", "-O1", "-g", "-fsanitize=address", "-fno-omit-frame-pointer", "-D_GNU_SOURCE",
+         f"-I{EMU}", f"-I{ROOT / 'include'}", f"-I{CSRC}"]
+
+CODES = {"oneTargGate": 0, "manyCtrlOneTargGate": 1, "swapGate": 2, "manyTargGate": 3, "pauliTensor": 4,
+         "pauliGadget": 5, "phaseGadget": 6}
+
+
+def _build():
+    BUILD.mkdir(exist_ok=True)
+    exe = BUILD / "emu_drive"
+    sources = [CSRC / f for f in ("kernels_tile.hip", "kernels_batch.hip", "engine.cpp", "circuit.cpp", "batch.cpp",
+                                  "engine.hpp", "tile_ops.hpp", "common.hpp", "launch.hpp")]
+    sources += [EMU / "emu_rt.cpp", EMU / "emu_drive.cpp", EMU / "hip" / "hip_runtime.h"]
+    if exe.exists() and all(s.stat().st_mtime < exe.stat().st_mtime for s in sources):
+        return exe
+    # the tile kernel's dynamic LDS becomes a static array (workgroups run one at a time)
+    kt = (CSRC / "kernels_tile.hip").read_text().replace(
+        "extern __shared__ unsigned char smem_raw[];",
+        "static unsigned char smem_raw[262144] __attribute__((aligned(16)));")
+    (BUILD / "kernels_tile_emu.cpp").write_text(kt)
+    units = [(BUILD / "kernels_tile_emu.cpp", []), (CSRC / "kernels_batch.hip", ["-x", "c++"]),
+             (CSRC / "engine.cpp", []), (CSRC / "circuit.cpp", []), (CSRC / "batch.cpp", []),
+             (EMU / "emu_rt.cpp", []), (EMU / "emu_drive.cpp", [])]
+    procs, objs = [], []
+    for src, extra in units:
+        obj = BUILD / (src.stem + ".o")
+        objs.append(obj)
+        procs.append(subprocess.Popen([CXX, *FLAGS, *extra, "-c", str(src), "-o", str(obj)],
+                                      stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True))
+    for p in procs:
+        out, _ = p.communicate()
+        if p.returncode != 0:
+            raise RuntimeError(out)
+    # host launchers of kernels not emulated here are never called by the driven paths
+    subprocess.run([CXX, "-fsanitize=address", *map(str, objs), "-o", str(exe), "-lpthread",
+                    "-Wl,--unresolved-symbols=ignore-all"], check=True)
+    return exe
+
+
+@pytest.fixture(scope="module")
+def drive():
+    if not (Path(CXX).exists() or shutil.which(CXX)):
+        pytest.skip("ROCm clang++ not installed")
+    return _build()
+
+
+def _run(exe, args, env=None):
+    e = dict(os.environ, ASAN_OPTIONS="detect_leaks=0")
+    e.update(env or {})
+    r = subprocess.run([str(exe), *map(str, args)], capture_output=True, text=True, env=e, timeout=900)
+    assert r.returncode == 0, r.stdout + r.stderr
+    vals = dict(kv.split("=") for kv in r.stdout.split() if "=" in kv)
+    return {k: float(v) for k, v in vals.items()}
+
+
+def _write_op(f, name, args):
+    code = CODES[name]
+    th, mat, p = 0.0, None, []
+    if code == 0:
+        q, mat = [args[0]], args[1]
+    elif code == 1:
+        q, mat = list(args[0]) + [args[1]], args[2]
+    elif code == 2:
+        q = [args[0], args[1]]
+    elif code == 3:
+        q, mat = list(args[0]), args[1]
+    elif code == 4:
+        q, p = list(args[0]), list(args[1])
+    elif code == 5:
+        q, p, th = list(args[0]), list(args[1]), args[2]
+    else:
+        q, th = list(args[0]), args[1]
+    np.array([code, len(q)] + q + p, np.uint32).tofile(f)
+    np.array([th], np.float64).tofile(f)
+    if mat is not None:
+        np.ascontiguousarray(np.asarray(mat, np.complex128)).tofile(f)
+
+
+def _unitary(rng, d):
+    u, _, vh = np.linalg.svd(rng.normal(size=(d, d)) + 1j * rng.normal(size=(d, d)))
+    return u @ vh
+
+
+def _random_op(rng, n, dense_only=False):
+    q = [int(x) for x in rng.permutation(n)]
+    kind = "many" if dense_only else ["one", "ctrl", "swap", "many", "tensor", "gadget", "phase"][int(rng.integers(7))]
+    if kind == "many":
+        k = int(rng.integers(3, 6)) if dense_only else int(rng.integers(2, 6))
+        return "manyTargGate", (q[:k], _unitary(rng, 1 << k))
+    if kind == "one":
+        return "oneTargGate", (q[0], _unitary(rng, 2))
+    if kind == "ctrl":
+        return "manyCtrlOneTargGate", (q[1:3], q[0], _unitary(rng, 2))
+    if kind == "swap":
+        return "swapGate", (q[0], q[1])
+    k = int(rng.integers(1, 5))
+    p = [int(x) for x in rng.integers(1, 4, k)]
+    if kind == "tensor":
+        return "pauliTensor", (q[:k], p)
+    if kind == "gadget":
+        return "pauliGadget", (q[:k], p, float(rng.uniform(0, 6.3)))
+    return "phaseGadget", (q[:k], float(rng.uniform(0, 6.3)))
+
+
+def _circuit_file(path, orc, n, ops, rng):
+    v = rng.normal(size=1 << n) + 1j * rng.normal(size=1 << n)
+    v /= np.linalg.norm(v)
+    ref = v.copy()
+    for name, args in ops:
+        getattr(orc, "sv_" + name)(ref, *args)
+    with open(path, "wb") as f:
+        np.array([n, len(ops)], np.uint32).tofile(f)
+        for name, args in ops:
+            _write_op(f, name, args)
+        v.astype(np.complex128).tofile(f)
+        ref.astype(np.complex128).tofile(f)
+
+
+@pytest.mark.parametrize("bits", ["12", "13"])
+@pytest.mark.parametrize("mfma", ["1", "0"])
+def test_dense_passes_on_cpu(drive, orc, tmp_path, bits, mfma):
+    rng = np.random.default_rng(31)
+    n = 14
+    path = tmp_path / "dense.bin"
+    _circuit_file(path, orc, n, [_random_op(rng, n, dense_only=True) for _ in range(24)], rng)
+    out = _run(drive, ["circuit", path], {"FSA_TILE_BITS": bits, "FSA_MFMA": mfma})
+    assert out["rel_err"] <= 1e-12, out
+
+
+def test_mixed_circuit_on_13_bit_tiles_on_cpu(drive, orc, tmp_path):
+    rng = np.random.default_rng(32)
+    n = 15
+    ops = [op for op in (_random_op(rng, n) for _ in range(80)) if op[0] != "manyCtrlOneTargGate"]
+    path = tmp_path / "mixed.bin"
+    _circuit_file(path, orc, n, ops, rng)
+    out = _run(drive, ["circuit", path], {"FSA_TILE_BITS": "13"})
+    assert out["rel_err"] <= 1e-12, out
+
+
+def _oracle_adjoint(orc, n, ops, coeffs, paulis):
+    psi = np.zeros(1 << n, complex)
+    psi[0] = 1
+    for name, a in ops:
+        getattr(orc, "sv_" + name)(psi, *a)
+    lam = orc.sv_applyPauliSum(psi, coeffs, paulis)
+    expec = orc.sv_innerProduct(psi, lam).real
+    grads, phi = [], psi
+    for name, a in reversed(ops):
+        if name == "pauliGadget":
+            t, p, th = a
+            pphi = phi.copy()
+            orc.sv_pauliTensor(pphi, t, p)
+            grads.append(-2 * np.vdot(lam, pphi).imag)
+            inv = (t, p, -th)
+        elif name == "oneTargGate":
+            inv = (a[0], a[1].conj().T)
+        elif name == "manyCtrlOneTargGate":
+            inv = (a[0], a[1], a[2].conj().T)
+        else:
+            inv = (a[0], a[1].conj().T)
+        getattr(orc, "sv_" + name)(phi, *inv)
+        getattr(orc, "sv_" + name)(lam, *inv)
+    return expec, grads[::-1]
+
+
+@pytest.mark.parametrize("prec,tol", [(0, 1e-12), (1, 2e-5)])
+def test_batch_kernel_on_cpu(drive, orc, tmp_path, prec, tol):
+    import importlib.util
+
+    spec = importlib.util.spec_from_file_location("fsa_amd_circuits",
+                                                  ROOT / "distributed-full-state-algorithms_amd" / "circuits.py")
+    circ = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(circ)
+    jobs = circ.config5(seed=41, num=5, nmin=10, nmax=14, num_terms=5)
+    path = tmp_path / "batch.bin"
+    with open(path, "wb") as f:
+        np.array([len(jobs)], np.uint32).tofile(f)
+        for n, ops, coeffs, paulis in jobs:
+            ops = ops[:40]
+            expec, grads = _oracle_adjoint(orc, n, ops, coeffs, paulis)
+            np.array([n, len(ops), len(coeffs)], np.uint32).tofile(f)
+            for name, args in ops:
+                _write_op(f, name, args)
+            np.asarray(coeffs, np.float64).tofile(f)
+            np.asarray(paulis, np.uint32).tofile(f)
+            np.array([expec] + grads, np.float64).tofile(f)
+    # tile bits 11: the 12..14-qubit jobs stream through LDS, the others stay in it
+    out = _run(drive, ["batch", path, 512, 11, 4, prec])
+    assert out["expec_abs_err"] <= tol and out["grad_rel_err"] <= tol, out
