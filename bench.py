@@ -32,6 +32,7 @@ import __graft_entry__ as ge  # noqa: E402
 
 METRIC = "HBM GB/s & gates/s @30q SV; adjoint grad circuits/s FP32 vs FP64 @ stated err"
 HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
 
 
 def log(msg):
@@ -238,6 +239,28 @@ def batch_bench(fsa, circ, reps=3, sample=512):
     return out
 
 
+def micro_passes(fsa, n, reps=3):
+    """Two single-pass workloads on an n-qubit FP64 state, `reps` launches each:
+    a pass holding one 5-target dense gate (MFMA tile variant) and a pass of
+    two phase gadgets (plain variant).  For hardware-counter runs."""
+    rng = np.random.default_rng(1)
+    z = rng.normal(size=(32, 32)) + 1j * rng.normal(size=(32, 32))
+    u, _, vh = np.linalg.svd(z)
+    hi = list(range(n - 7, n))
+    psi = fsa.StateVector(n)
+    psi.init_gaussian(3)
+    dense = fsa.Circuit([("manyTargGate", (hi[:5], u @ vh)), ("phaseGadget", ([1, 2], 0.3))])
+    plain = fsa.Circuit([("phaseGadget", ([1, 20], 0.3)), ("phaseGadget", ([3, 22], 0.4))])
+    fsa.profile_enable(True)
+    for c in (dense, plain):
+        for _ in range(reps):
+            c.apply(psi)
+        psi.sync()
+    prof = fsa.profile_read()
+    fsa.profile_enable(False)
+    print(json.dumps({k: {"launches": v[0], "mean_ms": round(v[1] / v[0], 4)} for k, v in prof.items()}), flush=True)
+
+
 def copy_calibration(n, reps=5):
     """A plain device-to-device copy of one n-qubit FP64 state (torch copy_),
     timed with events: the practical streaming ceiling next to the 8 TB/s peak,
@@ -275,6 +298,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only time a plain copy of one state-sized buffer and exit")
+    ap.add_argument("--micro-passes", action="store_true",
+                    help="only run one dense-gate pass and one diagonal pass (3 launches each) and exit")
     a = ap.parse_args()
 
     if a.gpus != 1:
@@ -287,6 +312,9 @@ def main():
         return
 
     fsa = ge.load_package()
+    if a.micro_passes:
+        micro_passes(fsa, a.qubits)
+        return
     circ = load_circuits_module()
     n = a.qubits
     seed = 20261204
@@ -335,6 +363,11 @@ def main():
         except Exception:
             traffic = None
 
+    # whole-step floor: every pass moves the state once at the HBM peak, and the
+    # dense gates of 3-5 targets need their FP64 MFMA time at the matrix peak
+    mfma_flops = sum(8 * (1 << len(a[0])) * (1 << n) for nm, a in ops if nm == "manyTargGate" and len(a[0]) >= 3)
+    hbm_floor_ms = passes * bytes_per_launch / (HBM_PEAK_GBS * 1e9) * 1e3
+    mfma_floor_ms = mfma_flops / (FP64_PEAK_TFLOPS * 1e12) * 1e3
     result = {
         "metric": METRIC,
         "value": round(gates_per_s, 2),
@@ -370,6 +403,9 @@ def main():
             "launches": dom_count,
             "mean_launch_ms": round(mean_ms, 4),
             "algorithmic_bytes_per_launch": bytes_per_launch,
+            "step_hbm_floor_ms": round(hbm_floor_ms, 1),
+            "step_mfma_floor_ms": round(mfma_floor_ms, 1),
+            "step_frac_of_floor": round(max(hbm_floor_ms, mfma_floor_ms) / ms_per_step, 4),
         },
         "kernel_time_ms": {k: round(v[1], 2) for k, v in prof.items()},
     }
