@@ -218,3 +218,106 @@ def algorithmic_bytes(name: str, args, nq: int, amp_bytes: int = 16, dm: bool = 
     if name == "swapGate":
         return full // 2
     return full
+
+
+# ---- on-disk circuit format ---------------------------------------------------
+# One .npz file (numpy's zip of .npy arrays; read with allow_pickle=False):
+#   "format"      -> "fsa-circuit-1"
+#   "header"      -> JSON text: {"num_qubits": n, "density_matrix": bool,
+#                    "ops": [{"name": ..., "qubits": [...], "controls": [...],
+#                             "paulis": [...], "theta": x, "prob": p,
+#                             "matrix": index, "kraus": index}, ...],
+#                    "observable": {"coeffs": index, "paulis": index} | null}
+#   "m<i>"        -> complex128 matrices (gate matrices, Kraus stacks), by index
+#   "coeffs<i>", "paulis<i>" -> observable arrays
+# Names and argument meaning are the reference's operation names
+# (src/distributed_statevector.hpp, src/distributed_densitymatrix.hpp).
+
+_ARGS = {
+    "oneTargGate": ("qubits", "matrix"), "manyCtrlOneTargGate": ("controls", "qubits", "matrix"),
+    "swapGate": ("qubits",), "manyTargGate": ("qubits", "matrix"), "pauliTensor": ("qubits", "paulis"),
+    "pauliGadget": ("qubits", "paulis", "theta"), "phaseGadget": ("qubits", "theta"),
+    "krausMap": ("kraus", "qubits"), "oneQubitDephasing": ("qubits", "prob"),
+    "twoQubitDephasing": ("qubits", "prob"), "oneQubitDepolarising": ("qubits", "prob"),
+    "twoQubitDepolarising": ("qubits", "prob"), "damping": ("qubits", "prob"),
+}
+
+
+def save_circuit(path, num_qubits: int, ops, density_matrix: bool = False, coeffs=None, paulis=None) -> None:
+    """Write a (name, args) op list, and optionally an observable, to `path` (.npz)."""
+    import json
+
+    arrays, recs = {}, []
+    for name, args in ops:
+        if name not in _ARGS:
+            raise ValueError(f"unknown operation {name}")
+        rec = {"name": name}
+        a = list(args)
+        fields = _ARGS[name]
+        if name in ("swapGate",):
+            rec["qubits"] = [int(a[0]), int(a[1])]
+        elif name in ("oneTargGate",):
+            rec["qubits"] = [int(a[0])]
+        elif name in ("twoQubitDephasing", "twoQubitDepolarising"):
+            rec["qubits"] = [int(a[0]), int(a[1])]
+            rec["prob"] = float(a[2])
+        elif name in ("oneQubitDephasing", "oneQubitDepolarising", "damping"):
+            rec["qubits"] = [int(a[0])]
+            rec["prob"] = float(a[1])
+        else:
+            for f, v in zip(fields, a):
+                if f in ("qubits", "controls", "paulis"):
+                    rec[f] = [int(x) for x in (v if not np.isscalar(v) else [v])]
+                elif f == "theta":
+                    rec[f] = float(v)
+        for f, v in zip(fields, a):
+            if f in ("matrix", "kraus"):
+                key = f"m{len(arrays)}"
+                arrays[key] = np.ascontiguousarray(np.asarray(v, dtype=np.complex128))
+                rec[f] = key
+        recs.append(rec)
+    obs = None
+    if coeffs is not None:
+        arrays["coeffs0"] = np.asarray(coeffs, dtype=np.float64)
+        arrays["paulis0"] = np.asarray(paulis, dtype=np.uint32)
+        obs = {"coeffs": "coeffs0", "paulis": "paulis0"}
+    header = {"num_qubits": int(num_qubits), "density_matrix": bool(density_matrix), "ops": recs, "observable": obs}
+    np.savez(path, format=np.array("fsa-circuit-1"), header=np.array(json.dumps(header)), **arrays)
+
+
+def load_circuit(path):
+    """(num_qubits, ops, density_matrix, coeffs or None, paulis or None) from save_circuit's file."""
+    import json
+
+    with np.load(path, allow_pickle=False) as z:
+        if str(z["format"]) != "fsa-circuit-1":
+            raise ValueError(f"{path} is not an fsa circuit file")
+        h = json.loads(str(z["header"]))
+        ops = []
+        for rec in h["ops"]:
+            name = rec["name"]
+            q = rec.get("qubits", [])
+            if name == "oneTargGate":
+                ops.append((name, (q[0], z[rec["matrix"]])))
+            elif name == "manyCtrlOneTargGate":
+                ops.append((name, (rec["controls"], q[0], z[rec["matrix"]])))
+            elif name == "swapGate":
+                ops.append((name, (q[0], q[1])))
+            elif name == "manyTargGate":
+                ops.append((name, (q, z[rec["matrix"]])))
+            elif name == "pauliTensor":
+                ops.append((name, (q, rec["paulis"])))
+            elif name == "pauliGadget":
+                ops.append((name, (q, rec["paulis"], rec["theta"])))
+            elif name == "phaseGadget":
+                ops.append((name, (q, rec["theta"])))
+            elif name == "krausMap":
+                ops.append((name, (z[rec["kraus"]], q)))
+            elif name in ("twoQubitDephasing", "twoQubitDepolarising"):
+                ops.append((name, (q[0], q[1], rec["prob"])))
+            else:
+                ops.append((name, (q[0], rec["prob"])))
+        obs = h.get("observable")
+        coeffs = z[obs["coeffs"]] if obs else None
+        paulis = z[obs["paulis"]] if obs else None
+    return h["num_qubits"], ops, h["density_matrix"], coeffs, paulis
