@@ -104,7 +104,8 @@ struct TileOp {
     uint32_t kind;
     uint32_t k;           // MATK: number of targets; SWEEP: number of micro-ops
     uint32_t p[6];        // positions (targets in caller order for item-form MATK, ascending in micro form)
-    uint32_t lmask;       // MAT1: control mask; PAULI: XY mask; PHASE: parity mask
+    uint32_t lmask;       // MAT1: control mask; PAULI: XY mask; PHASE: parity mask;
+                          // MATK (MFMA form): offset (reals) of the matrix in the pass's LDS area
     uint32_t lmask2;      // PAULI: YZ mask
     uint64_t gmask;       // MAT1: out-of-tile control mask; PAULI/PHASE: out-of-tile parity mask
     uint64_t gq;          // micro form: the same masks restricted to tile bits outside the sweep
@@ -115,6 +116,10 @@ struct TileOp {
 static_assert(sizeof(TileOp) % 8 == 0, "TileOp must stay 8-byte aligned");
 
 constexpr uint32_t NO_PARAM = 0xffffffffu;
+constexpr uint32_t MAT_UNPLACED = 0xffffffffu;  // MFMA-form MATK item not yet given an LDS offset
+// LDS bytes per workgroup for MFMA matrices: two 64 KiB tiles plus two such
+// areas fill the 160 KiB of LDS of a CU
+constexpr uint32_t TILE_MAT_LDS_BYTES = 16384;
 constexpr uint32_t DIAG_RUN_MAX = 32;      // items per OP_DIAG_RUN
 constexpr uint32_t DIAG_RUN_GRADS = 6;     // gradient items per OP_DIAG_RUN
 constexpr int TILE_MAX_BITS = 14;

@@ -373,6 +373,7 @@ TileOp translate(const HOp& op, const int* pos, uint64_t tileMask, std::vector<c
             // MFMA form: real 2D x 2D matrix [[Ur, -Ui], [Ui, Ur]], row-major, in
             // complex slots of the parameter buffer (two reals per slot)
             t.lmask2 = 1;
+            t.lmask = MAT_UNPLACED;         // the pass gives it an LDS offset
             const uint64_t D = 1ULL << k, D2 = 2 * D;
             std::vector<R> real(D2 * D2);
             for (uint64_t r = 0; r < D; r++)
@@ -617,7 +618,7 @@ void execute_impl(State& s, const std::vector<HOp>& ops, const std::vector<Stage
     std::vector<TileOp> items, micro;
     std::vector<cx<R>> mats;
     std::vector<size_t> first(stages.size(), 0), count(stages.size(), 0);
-    std::vector<uint32_t> flags(stages.size(), 0);
+    std::vector<uint32_t> flags(stages.size(), 0), matLds(stages.size(), 0);
     std::vector<TileGeom> geoms(stages.size());
     for (size_t si = 0; si < stages.size(); si++) {
         const Stage& st = stages[si];
@@ -646,18 +647,25 @@ void execute_impl(State& s, const std::vector<HOp>& ops, const std::vector<Stage
             const HOp& op = ops[oi];
             if ((op.kind == OP_MATK && !sweepable(op, W)) || op.kind == OP_DEPOL2) denseStage = true;
         }
-        // the MFMA kernel variant carries no VALU dense gates: a pass uses it only
+        // the MFMA kernel variants carry no VALU dense gates: a pass uses one only
         // when every dense gate in it has the MFMA form (k = 3..5 on a full tile)
         const bool useMfma = env_flag("FSA_MFMA", true);
         bool stageMfma = useMfma && m >= FULL;        // full tiles of 2^12 (FP64) / 2^13 amplitudes or more
         bool anyMfma = false;
+        size_t mfmaBytes = 0;
         for (int oi : st.ops) {
             const HOp& op = ops[oi];
             if (op.kind != OP_MATK) continue;
-            if (op.q.size() >= 3 && op.q.size() <= 5) anyMfma = true;
-            else if (op.q.size() != 2) stageMfma = false;   // two-target gates run on VALU in either variant
+            if (op.q.size() >= 3 && op.q.size() <= 5) {
+                anyMfma = true;
+                mfmaBytes += (2 * sizeof(R)) << (2 * op.q.size());
+            } else if (op.q.size() != 2) {
+                stageMfma = false;   // two-target gates run on VALU in either variant
+            }
         }
         stageMfma = stageMfma && anyMfma;
+        // MFMA matrices sit in LDS (2 * 4^k reals each) when they fit in its area
+        const bool matsInLds = stageMfma && mfmaBytes <= TILE_MAT_LDS_BYTES;
         for (int oi : st.ops) {
             const HOp& op = ops[oi];
             if (useSweeps && !denseStage && m >= W && sweepable(op, W)) {
@@ -694,8 +702,23 @@ void execute_impl(State& s, const std::vector<HOp>& ops, const std::vector<Stage
             for (int oi : gr.ops) micro.push_back(translate_micro<R>(ops[oi], pos, sl, mats));
             items.push_back(it);
         }
-        fuse_diag_runs(items, first[si], false);
+        if (!stageMfma) fuse_diag_runs(items, first[si], false);
         count[si] = items.size() - first[si];
+        // every MFMA matrix of the pass goes to LDS (the left half of the real
+        // form, 2 * 4^k reals)
+        if (matsInLds) {
+            uint32_t used = 0;
+            for (size_t i = first[si]; i < items.size(); i++) {
+                TileOp& it = items[i];
+                if (it.kind != OP_MATK || it.lmask2 != 1u) continue;
+                const uint32_t need = (uint32_t)(2u * sizeof(R)) << (2 * it.k);
+                if (used + need > TILE_MAT_LDS_BYTES)
+                    fail(FSA_ERR_INTERNAL, "MFMA matrices of a tile pass exceed its LDS budget");
+                it.lmask = used / (uint32_t)sizeof(R);
+                used += need;
+            }
+            matLds[si] = used;
+        }
         // fuse the first / last sweep with the HBM load / store on full tiles when
         // its bits leave the 3 lowest tile bits to the lanes (8 consecutive amplitudes)
         auto fusable = [&](const TileOp& it) { return it.kind == OP_SWEEP && m == FULL && it.p[0] >= 3; };
@@ -704,7 +727,8 @@ void execute_impl(State& s, const std::vector<HOp>& ops, const std::vector<Stage
             if (fusable(items[first[si] + count[si] - 1])) flags[si] |= 2u;
             if (count[si] == 1 && flags[si] != 3u) flags[si] = 0;
         }
-        // kernel variant: 1 for dense gates / two-qubit depolarising; bit 4 marks sweeps
+        // kernel variant: 1 for dense gates / two-qubit depolarising, 2 for MFMA
+        // dense gates with matrices in LDS, 3 with matrices in HBM; bit 4 marks sweeps
         uint32_t var = 0;
         bool hasSweep = false;
         for (size_t i = first[si]; i < first[si] + count[si]; i++) {
@@ -712,8 +736,9 @@ void execute_impl(State& s, const std::vector<HOp>& ops, const std::vector<Stage
             if (items[i].kind == OP_DEPOL2) var = std::max(var, 1u);
             if (items[i].kind == OP_SWEEP) hasSweep = true;
         }
+        if (var == 2u && !matsInLds) var = 3u;
         for (size_t i = first[si]; i < first[si] + count[si]; i++)
-            if (items[i].kind == OP_MATK && (items[i].lmask2 == 1u) != (var == 2u) && !(var == 2u && items[i].k == 2))
+            if (items[i].kind == OP_MATK && (items[i].lmask2 == 1u) != (var >= 2u) && !(var >= 2u && items[i].k == 2))
                 fail(FSA_ERR_INTERNAL, "tile pass mixes MFMA and VALU dense gates");
         flags[si] |= var << 2;
         if (hasSweep) flags[si] |= 16u;
@@ -741,10 +766,10 @@ void execute_impl(State& s, const std::vector<HOp>& ops, const std::vector<Stage
             // one profile name per kernel instantiation family, so the timings
             // line up with per-kernel profiler statistics
             const uint32_t var = (flags[si] >> 2) & 3u;
-            ProfScope prof(var == 2 ? "tile_pass_mfma" : var == 1 ? "tile_pass_dense"
+            ProfScope prof(var == 3 ? "tile_pass_mfma_hbm" : var == 2 ? "tile_pass_mfma" : var == 1 ? "tile_pass_dense"
                            : (flags[si] & 16u) ? "tile_pass_sweep" : "tile_pass_plain", s.stream);
             launch_tile<R>(s.stream, static_cast<cx<R>*>(s.amps), s.nq, geoms[si], ditems + first[si],
-                           (unsigned)count[si], dmicro, dmats, flags[si]);
+                           (unsigned)count[si], dmicro, dmats, flags[si], matLds[si]);
             hip_check(hipGetLastError(), "tile kernel launch");
         } else {
             for (int oi : st.ops) run_standalone<R>(s, ops[oi]);

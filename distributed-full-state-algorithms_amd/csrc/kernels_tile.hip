@@ -13,6 +13,8 @@
 #include "tile_ops.hpp"
 
 #include <algorithm>
+#include <mutex>
+#include <unordered_map>
 
 namespace fsa {
 
@@ -299,18 +301,23 @@ __device__ __forceinline__ uint64_t tile_scatter(uint32_t l, const TileGeom& g) 
 
 // Persistent, software-pipelined pass.  A workgroup walks tiles blockIdx.x,
 // blockIdx.x + gridDim.x, ...; while the items of tile t run, the loads of the
-// next tile are in flight into registers (barriers on gfx950 wait only for LDS
-// traffic, so the prefetch is not drained by them).  FIN: the first item is a
+// next tile (into registers) and the write-back of the previous one are in
+// flight.  Barriers on gfx950 wait only for LDS traffic, and the items issue
+// no vector-memory loads (MFMA matrices are staged in LDS), so nothing waits
+// for that traffic before the items end.  FIN: the first item is a
 // sweep fused with the HBM load (registers in that sweep's layout); FOUT: the
 // last item is a sweep fused with the HBM store.  A pass made of a single
 // fused sweep never touches LDS.  ROWS = tile size / TB for row-layout loads.
+// s_waitcnt operand for vmcnt(0) with expcnt and lgkmcnt left at their maxima
+// (gfx9 encoding: vmcnt bits 3:0 and 15:14, expcnt 6:4, lgkmcnt 11:8)
+constexpr int WAIT_VMCNT0 = 0x0F70;
+
 template <typename R, int TB, int ROWS, int W, int VAR, bool SW>
 __global__ __launch_bounds__(TB, (sizeof(R) == 8 && ROWS >= 16) ? 2 : 4) void k_tile(cx<R>* __restrict__ a, TileGeom geom,
                                                 const TileOp* __restrict__ items, unsigned numItems,
                                                 const TileOp* __restrict__ micro, const cx<R>* __restrict__ mats,
                                                 uint64_t numTiles, uint32_t flags) {
     extern __shared__ unsigned char smem_raw[];
-    __shared__ DiagEnt dtab[DIAG_RUN_MAX];
     cx<R>* s = reinterpret_cast<cx<R>*>(smem_raw);
     constexpr int LOGTB = TB == 1024 ? 10 : TB == 512 ? 9 : TB == 256 ? 8 : 7;
     constexpr int LOGROWS = ROWS >= 16 ? 4 : ROWS >= 8 ? 3 : ROWS >= 4 ? 2 : ROWS >= 2 ? 1 : 0;
@@ -359,37 +366,68 @@ __global__ __launch_bounds__(TB, (sizeof(R) == 8 && ROWS >= 16) ? 2 : 4) void k_
         }                                                                                               \
     } while (0)
 
-    // Prefetch loads are unconditional (keeping the buffer in registers):
-    // threads beyond a small tile repeat valid addresses of it, and the last
-    // prefetch of a workgroup re-reads its current tile.
+    // Threads beyond a small tile load valid addresses of it (repeats) and
+    // neither store nor write LDS.  Sweep passes prefetch unconditionally: the
+    // last prefetch of a workgroup re-reads its current tile.
     R bx[NB], by[NB];
     uint64_t tile = blockIdx.x;
     if (tile >= numTiles) return;
-    FSA_PREFETCH(tile);
-    for (; tile < numTiles; tile += gridDim.x) {
-        const uint64_t base = ins0_mask(tile, geom.tileMask);
-        const uint64_t next = tile + gridDim.x < numTiles ? tile + gridDim.x : tile;
-        if (!fin) {
-            if (owner) {
-#pragma unroll
-                for (int r = 0; r < ROWS; r++) s[((uint32_t)r << LOGTB) | tid] = mk<R>(bx[r], by[r]);
+    // MFMA matrices with an LDS offset are copied once, behind the tile, in the
+    // order mat_lds_index reads them (the left half of the real form)
+    R* lmat = reinterpret_cast<R*>(smem_raw + ((size_t)sizeof(cx<R>) << m));
+    if constexpr (VAR == 2) {
+        for (unsigned it = 0; it < numItems; it++) {
+            const TileOp& op = items[it];
+            if (op.kind != OP_MATK || op.lmask2 != 1u || op.lmask == MAT_UNPLACED) continue;
+            const uint32_t k = op.k, D = 1u << k;
+            const R* M = reinterpret_cast<const R*>(mats + op.mat_off);
+            for (uint32_t e = tid; e < 2 * D * D; e += TB) {
+                const uint32_t r = e >> k, c = e & (D - 1);
+                lmat[op.lmask + r * D + (c ^ (r & (D - 1)))] = M[(size_t)r * 2 * D + c];
             }
-            __syncthreads();
-            FSA_PREFETCH(next);
+        }
+        __syncthreads();
+    }
+    FSA_PREFETCH(tile);
+    uint64_t next = tile + gridDim.x;
+    if constexpr (!SW) {
+        if (owner) {
+#pragma unroll
+            for (int r = 0; r < ROWS; r++) s[((uint32_t)r << LOGTB) | tid] = mk<R>(bx[r], by[r]);
+        }
+        __syncthreads();
+        if (next < numTiles) FSA_PREFETCH(next);
+    }
+    for (;;) {
+        const uint64_t base = ins0_mask(tile, geom.tileMask);
+        // sweep passes: the prefetch target, the current tile when it is the last
+        const uint64_t nx = next < numTiles ? next : tile;
+        if constexpr (SW) {
+            if (!fin) {
+                if (owner) {
+#pragma unroll
+                    for (int r = 0; r < ROWS; r++) s[((uint32_t)r << LOGTB) | tid] = mk<R>(bx[r], by[r]);
+                }
+                __syncthreads();
+                FSA_PREFETCH(nx);
+            }
         }
         for (unsigned it = 0; it < numItems; it++) {
             const TileOp& op = items[it];
-            if (op.kind == OP_DIAG_RUN) {
-                const uint32_t cnt = op.k;
-                if (tid < cnt) dtab[tid] = diag_entry(items[it + 1 + tid], base);
-                __syncthreads();
-                diag_sweep<R, TB, false, 0>(s, nullptr, T, dtab, cnt, nullptr, 0, nullptr, nullptr);
-                __syncthreads();
-                it += cnt;
-                continue;
+            if constexpr (VAR < 2) {   // the host builds no diagonal runs in MFMA passes
+                if (op.kind == OP_DIAG_RUN) {
+                    __shared__ DiagEnt dtab[DIAG_RUN_MAX];
+                    const uint32_t cnt = op.k;
+                    if (tid < cnt) dtab[tid] = diag_entry(items[it + 1 + tid], base);
+                    __syncthreads();
+                    diag_sweep<R, TB, false, 0>(s, nullptr, T, dtab, cnt, nullptr, 0, nullptr, nullptr);
+                    __syncthreads();
+                    it += cnt;
+                    continue;
+                }
             }
             if (!SW || op.kind != OP_SWEEP) {
-                tile_apply<R, TB, VAR, (ROWS * TB >= 8192 ? ROWS * TB / 4096 : 1)>(s, T, base, op, mats);
+                tile_apply<R, TB, VAR, (ROWS * TB >= 8192 ? ROWS * TB / 4096 : 1)>(s, T, base, op, mats, lmat);
                 __syncthreads();
                 continue;
             }
@@ -425,14 +463,46 @@ __global__ __launch_bounds__(TB, (sizeof(R) == 8 && ROWS >= 16) ? 2 : 4) void k_
                     for (int i = 0; i < G; i++) s[gb | sw_dep<W>(i, op.p)] = v[i];
                 }
             }
-            if (fromRegs) FSA_PREFETCH(next);
+            if (fromRegs) FSA_PREFETCH(nx);
             if (!toHbm) __syncthreads();
         }
-        if (!fout && owner) {
+        if constexpr (SW) {
+            if (!fout && owner) {
 #pragma unroll
-            for (int r = 0; r < ROWS; r++) a[base | FSA_ROWADDR(r)] = s[((uint32_t)r << LOGTB) | tid];
+                for (int r = 0; r < ROWS; r++) a[base | FSA_ROWADDR(r)] = s[((uint32_t)r << LOGTB) | tid];
+            }
+            __syncthreads();
+        } else {
+            // The write-back of this tile and the loads of the tile after the next
+            // are issued together, so both complete while the next tile's items
+            // run.  (Vector memory operations complete in order: stores issued
+            // just before the next tile is moved into LDS would be waited for
+            // there, leaving their latency exposed on every tile.)
+            // Each thread moves its own LDS slots out (to HBM) and the next
+            // tile in (from its registers), so no barrier separates the two;
+            // the wait below makes the next tile's loads land before the
+            // first store of this tile is issued.
+            __builtin_amdgcn_s_waitcnt(WAIT_VMCNT0);
+            const bool more = next < numTiles;
+            if (owner && more) {
+#pragma unroll
+                for (int r = 0; r < ROWS; r++) {
+                    const uint32_t l = ((uint32_t)r << LOGTB) | tid;
+                    const R vx = s[l].x, vy = s[l].y;
+                    s[l] = mk<R>(bx[r], by[r]);
+                    a[base | FSA_ROWADDR(r)] = mk<R>(vx, vy);
+                }
+            } else if (owner) {
+#pragma unroll
+                for (int r = 0; r < ROWS; r++) a[base | FSA_ROWADDR(r)] = s[((uint32_t)r << LOGTB) | tid];
+            }
+            if (!more) break;
+            if (next + gridDim.x < numTiles) FSA_PREFETCH(next + gridDim.x);
+            __syncthreads();   // the next tile is in LDS
         }
-        __syncthreads();
+        if (next >= numTiles) break;
+        tile = next;
+        next += gridDim.x;
     }
 #undef FSA_ROWADDR
 #undef FSA_SWADDR
@@ -443,7 +513,7 @@ __global__ __launch_bounds__(TB, (sizeof(R) == 8 && ROWS >= 16) ? 2 : 4) void k_
 
 template <typename R>
 void launch_tile(hipStream_t s, cx<R>* a, unsigned nq, const TileGeom& geom, const TileOp* ditems,
-                 unsigned numItems, const TileOp* dmicro, const cx<R>* dmats, uint32_t flags) {
+                 unsigned numItems, const TileOp* dmicro, const cx<R>* dmats, uint32_t flags, uint32_t matLds) {
     constexpr int TB = 512;
     constexpr int W = sweep_bits<R>();
     static int numCU = 0;
@@ -454,16 +524,20 @@ void launch_tile(hipStream_t s, cx<R>* a, unsigned nq, const TileGeom& geom, con
             numCU = 256;
     }
     uint64_t numTiles = 1ULL << (nq - geom.m);
-    size_t lds = sizeof(cx<R>) << geom.m;
-    // enough resident workgroups to fill every CU: two 64 KiB tiles per CU, one 128 KiB tile
+    const uint32_t var = (flags >> 2) & 3u;
+    if (matLds > (var == 2 ? TILE_MAT_LDS_BYTES : 0u))
+        fail(FSA_ERR_INTERNAL, "tile pass asks for more LDS for matrices than its variant has");
+    const size_t lds = (sizeof(cx<R>) << geom.m) + matLds;
+    // enough resident workgroups to fill every CU: two 64 KiB tiles (each with up
+    // to 16 KiB of MFMA matrices) per CU, one 128 KiB tile
     const uint64_t perCU = lds > 80 * 1024 ? 1 : 2;
     uint64_t grid = std::min<uint64_t>(numTiles, (uint64_t)numCU * perCU);
     const uint32_t rows = geom.m > 9 ? 1u << (geom.m - 9) : 1u;
-    const uint32_t var = (flags >> 2) & 3u;
     const bool sw = (flags & 16u) != 0;
     auto* kern = k_tile<R, TB, 1, W, 0, false>;
 #define FSA_PICK(RW)                                                                              \
-    kern = var == 2 ? k_tile<R, TB, RW, W, 2, false> : var == 1 ? k_tile<R, TB, RW, W, 1, false>          \
+    kern = var == 3 ? k_tile<R, TB, RW, W, 3, false> : var == 2 ? k_tile<R, TB, RW, W, 2, false>          \
+         : var == 1 ? k_tile<R, TB, RW, W, 1, false>                                                 \
          : sw ? k_tile<R, TB, RW, W, 0, true> : k_tile<R, TB, RW, W, 0, false>;
     switch (rows) {
     case 1: FSA_PICK(1) break;
@@ -475,17 +549,24 @@ void launch_tile(hipStream_t s, cx<R>* a, unsigned nq, const TileGeom& geom, con
 #undef FSA_PICK
     if (lds > 64 * 1024) {
         // dynamic LDS beyond 64 KiB must be granted per kernel before the launch
-        if (hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)lds) != hipSuccess)
-            fail(FSA_ERR_HIP, "cannot grant 128 KiB of LDS to the tile kernel");
+        static std::mutex mu;
+        static std::unordered_map<const void*, size_t> granted;
+        std::lock_guard<std::mutex> lock(mu);
+        size_t& g = granted[reinterpret_cast<const void*>(kern)];
+        if (lds > g) {
+            if (hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)lds) != hipSuccess)
+                fail(FSA_ERR_HIP, "cannot grant the tile kernel more than 64 KiB of LDS");
+            g = lds;
+        }
     }
     hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(TB), lds, s, a, geom, ditems, numItems, dmicro, dmats,
                        numTiles, flags);
 }
 
 template void launch_tile<double>(hipStream_t, cx<double>*, unsigned, const TileGeom&, const TileOp*, unsigned,
-                                  const TileOp*, const cx<double>*, uint32_t);
+                                  const TileOp*, const cx<double>*, uint32_t, uint32_t);
 template void launch_tile<float>(hipStream_t, cx<float>*, unsigned, const TileGeom&, const TileOp*, unsigned,
-                                 const TileOp*, const cx<float>*, uint32_t);
+                                 const TileOp*, const cx<float>*, uint32_t, uint32_t);
 
 }  // namespace fsa

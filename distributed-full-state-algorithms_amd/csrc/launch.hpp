@@ -40,10 +40,12 @@ void launch_scale(hipStream_t s, cx<R>* a, uint64_t num, R f);
 // items: plain ops or OP_SWEEP items whose micro-ops live in dmicro; flags bit 0:
 // the first item is a sweep fused with the HBM load, bit 1: the last with the
 // store, bits 2-3: kernel variant (0 simple ops, 1 adds VALU dense gates and
-// two-qubit depolarising, 2 MFMA dense gates), bit 4: the pass has sweeps
+// two-qubit depolarising, 2 MFMA dense gates with matrices in LDS, 3 with
+// matrices in HBM), bit 4: the pass has sweeps.
+// matLds: bytes of LDS after the tile for MFMA matrices (variant 2 only).
 template <typename R>
 void launch_tile(hipStream_t s, cx<R>* a, unsigned nq, const TileGeom& geom, const TileOp* ditems,
-                 unsigned numItems, const TileOp* dmicro, const cx<R>* dmats, uint32_t flags);
+                 unsigned numItems, const TileOp* dmicro, const cx<R>* dmats, uint32_t flags, uint32_t matLds = 0);
 // largest tile (log2 amplitudes) the LDS budget allows for this precision
 template <typename R> constexpr int tile_bits_max() { return sizeof(R) == 8 ? 12 : 13; }
 // largest tile a pass may use when asked for (FSA_TILE_BITS): 128 KiB of LDS

@@ -150,7 +150,21 @@ template <> struct MfmaAcc<float> {
 // A tile of T amplitudes holds T / 512 wave tasks for every K (RB * groups / 64),
 // so each of the TB / 64 waves takes MAXT = T / (8 TB) of them: 1 for 4096
 // amplitudes at TB = 512, 2 for 8192.
-template <typename R, int K, int TB, int MAXT>
+//
+// The A operand (the real matrix [[Ur, -Ui], [Ui, Ur]]) is read from LDS
+// (LDSA: M holds its left half, the 2D x D reals [Ur; Ui], stored by
+// mat_lds_index) or from HBM (M is the row-major 2D x 2D real form).  LDS
+// reads keep the item loop free of vector-memory loads, whose waits would
+// also wait for the next tile's prefetch (loads complete in order); HBM reads
+// serve passes whose matrices do not fit in the LDS area.
+template <int D>
+__device__ __forceinline__ uint32_t mat_lds_index(uint32_t r, uint32_t c) {
+    // columns XOR-swizzled by the row: the 16 rows a wave reads at one column
+    // fall in distinct banks
+    return r * (uint32_t)D + (c ^ (r & (uint32_t)(D - 1)));
+}
+
+template <typename R, int K, int TB, int MAXT, bool LDSA>
 __device__ __forceinline__ void tile_matk_mfma(cx<R>* s, uint32_t T, const TileOp& op, const R* __restrict__ M) {
     typedef typename MfmaAcc<R>::T acc_t;
     constexpr int D = 1 << K, D2 = 2 * D;
@@ -174,21 +188,39 @@ __device__ __forceinline__ void tile_matk_mfma(cx<R>* s, uint32_t T, const TileO
         for (int c = 0; c < 4; c++) acc[t][c] = acc_t{0, 0, 0, 0};
         if (taskId[t] >= tasks) continue;
         const uint32_t rb = taskId[t] % RB, cbase = (taskId[t] / RB) * 4;
-        const R* arow = M + (size_t)(rb * 16 + lr) * D2;
+        const uint32_t arow = rb * 16 + lr;                // real row of A this lane feeds
         // B-fragment group bases of the four column blocks for this lane
         uint32_t gb[4];
 #pragma unroll
         for (int c = 0; c < 4; c++) gb[c] = ins0_mask32((cbase + (uint32_t)c) * 16 + lr, tm);
         const R* sr = reinterpret_cast<const R*>(s);
-        // four k-steps per iteration: their A loads and LDS reads issue together
-#pragma unroll 4
-        for (int ks = 0; ks < D2 / 4; ks++) {
-            const R a = arow[ks * 4 + lk];
+        auto kstep = [&](int ks, R a) {
             const uint32_t r = (uint32_t)ks * 4 + lk;          // real row of X this lane feeds
             const uint32_t off = mat_offset<K>(r & (D - 1), b0, b1, b2, b3, b4);
             const uint32_t half = r < (uint32_t)D ? 0u : 1u;  // real or imaginary part only
 #pragma unroll
             for (int c = 0; c < 4; c++) acc[t][c] = MfmaAcc<R>::step(a, sr[2 * (gb[c] | off) + half], acc[t][c]);
+        };
+        // not unrolled: four independent 64-cycle FP64 MFMAs per k-step cover
+        // the operand latency, and the loop stays clear of register spills
+        if constexpr (LDSA) {
+            // columns C < D: A[row][C] = left[row][C]; columns C >= D:
+            // A[row][C] = -left[row + D][C - D] (row < D, the -Ui block) or
+            // left[row - D][C - D] (row >= D, the Ur block)
+            const uint32_t r2 = (arow + D) & (D2 - 1);
+            const R sgn = arow < (uint32_t)D ? (R)-1 : (R)1;
+#pragma unroll 1
+            for (int ks = 0; ks < D / 4; ks++) kstep(ks, M[mat_lds_index<D>(arow, (uint32_t)ks * 4 + lk)]);
+#pragma unroll 1
+            for (int ks = D / 4; ks < D2 / 4; ks++)
+                kstep(ks, sgn * M[mat_lds_index<D>(r2, (uint32_t)ks * 4 + lk - D)]);
+        } else {
+            // HBM operands: FP64 unrolled so the loads of two k-steps are in
+            // flight (FP32 tiles hold twice the accumulators: no registers left)
+            const R* ar = M + (size_t)arow * D2;
+            constexpr int HU = sizeof(R) == 8 ? 2 : 1;
+#pragma unroll HU
+            for (int ks = 0; ks < D2 / 4; ks++) kstep(ks, ar[ks * 4 + lk]);
         }
     }
     __syncthreads();
@@ -213,11 +245,13 @@ __device__ __forceinline__ void tile_matk_mfma(cx<R>* s, uint32_t T, const TileO
 
 // Kernel variants: VAR 0 carries the simple LDS ops (and, with SW, register
 // sweeps); VAR 1 adds VALU dense gates and two-qubit depolarising; VAR 2 has
-// MFMA dense gates instead of VALU ones.  Each instantiation stays within 128
-// VGPRs so two 64 KiB workgroups share a CU.
+// MFMA dense gates (matrices in LDS) instead of VALU ones, VAR 3 the same with
+// the matrices read from HBM.  Each instantiation stays within 128 VGPRs so
+// two 64 KiB workgroups share a CU.
+// lmat: the pass's LDS matrix area (VAR 2: every MFMA item has an offset in it)
 template <typename R, int TB, int VAR, int MAXT = 1>
 __device__ __forceinline__ void tile_apply(cx<R>* s, uint32_t T, uint64_t base, const TileOp& op,
-                           const cx<R>* __restrict__ mats) {
+                           const cx<R>* __restrict__ mats, const R* lmat = nullptr) {
     const uint32_t tid = threadIdx.x;
     switch (op.kind) {
     case OP_MAT1: {
@@ -279,16 +313,17 @@ __device__ __forceinline__ void tile_apply(cx<R>* s, uint32_t T, uint64_t base, 
         if constexpr (VAR == 0) return;
         const cx<R>* mat = mats + op.mat_off;
         if (op.lmask2 == 1u) {        // host chose the MFMA form (full tile, k = 3, 4 or 5)
-            if constexpr (VAR == 2) {
-                const R* M = reinterpret_cast<const R*>(mat);
-                if (op.k == 3) tile_matk_mfma<R, 3, TB, MAXT>(s, T, op, M);
-                else if (op.k == 4) tile_matk_mfma<R, 4, TB, MAXT>(s, T, op, M);
-                else tile_matk_mfma<R, 5, TB, MAXT>(s, T, op, M);
+            if constexpr (VAR >= 2) {
+                constexpr bool LDSA = VAR == 2;   // VAR 2: every MFMA matrix of the pass is in LDS
+                const R* M = LDSA ? lmat + op.lmask : reinterpret_cast<const R*>(mat);
+                if (op.k == 3) tile_matk_mfma<R, 3, TB, MAXT, LDSA>(s, T, op, M);
+                else if (op.k == 4) tile_matk_mfma<R, 4, TB, MAXT, LDSA>(s, T, op, M);
+                else tile_matk_mfma<R, 5, TB, MAXT, LDSA>(s, T, op, M);
             }
             return;
         }
-        if constexpr (VAR == 2) {
-            // the MFMA variant also carries two-target gates (VALU, in place)
+        if constexpr (VAR >= 2) {
+            // the MFMA variants also carry two-target gates (VALU, in place)
             if (op.k == 2) tile_matk<R, 2, TB>(s, T, op, mat);
             return;
         }

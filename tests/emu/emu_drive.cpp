@@ -1,6 +1,6 @@
 // Drives libfsa's real planner and kernel code on the CPU stand-in.
 //
-//   emu_drive circuit <file>              fused circuit (make_plan + execute + k_tile)
+//   emu_drive circuit <file>              fused circuit (make_plan + execute: k_tile, streaming kernels)
 //   emu_drive batch <file> TB BITS LOW P  batch (fsa_batch_* + k_batch), P = 0 FP64 / 1 FP32
 //
 // Input files are written by tests/test_emu_kernels.py.  Output: one line with
@@ -69,16 +69,20 @@ int run_circuit(const char* path) {
     s.amps = std::malloc(N * sizeof(cx<double>));
     std::memcpy(s.amps, v.data(), N * sizeof(cx<double>));
     Plan p = make_plan(c->c.ops, n, FSA_FP64);
-    for (const Stage& st : p.stages)
-        if (!st.tiled) { std::printf("untiled pass in the plan (not emulated)\n"); return 4; }
+    size_t tiled = 0;
+    for (const Stage& st : p.stages) tiled += st.tiled;
+    profile_enable(true);
     execute(s, p);
+    // launches per kernel family (the profiler's names), e.g. tile_pass_mfma=3
+    for (const ProfEntry& e : profile_read()) std::printf("%s=%llu ", e.name.c_str(), (unsigned long long)e.count);
+    profile_enable(false);
     double num = 0, den = 0;
     for (size_t i = 0; i < N; i++) {
         const cx<double> a = static_cast<cx<double>*>(s.amps)[i];
         num += (a.x - want[i].x) * (a.x - want[i].x) + (a.y - want[i].y) * (a.y - want[i].y);
         den += want[i].x * want[i].x + want[i].y * want[i].y;
     }
-    std::printf("passes=%zu rel_err=%.3e\n", p.stages.size(), std::sqrt(num / den));
+    std::printf("passes=%zu tiled=%zu rel_err=%.3e\n", p.stages.size(), tiled, std::sqrt(num / den));
     std::free(s.amps);
     fsa_circuit_destroy(c);
     return 0;

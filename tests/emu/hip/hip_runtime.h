@@ -29,6 +29,7 @@
 #define __launch_bounds__(...)
 #define __shared__ static
 #define __builtin_amdgcn_readfirstlane(x) (x)
+#define __builtin_amdgcn_s_waitcnt(x) ((void)(x))   // memory operations complete at once here
 #define HIP_SYMBOL(x) (&x)
 
 struct dim3 {
@@ -99,7 +100,11 @@ inline hipError_t hipStreamSynchronize(hipStream_t) { return hipSuccess; }
 inline hipError_t hipGetLastError() { return hipSuccess; }
 inline const char* hipGetErrorString(hipError_t) { return "cpu emulation"; }
 inline hipError_t hipGetDevice(int* d) { *d = 0; return hipSuccess; }
-inline hipError_t hipDeviceGetAttribute(int* v, int, int) { *v = 4; return hipSuccess; }
+// one CU: persistent grids hold two workgroups, so each walks several tiles
+inline hipError_t hipDeviceGetAttribute(int* v, int attr, int) {
+    *v = attr == hipDeviceAttributeMultiprocessorCount ? 1 : 4;
+    return hipSuccess;
+}
 inline hipError_t hipOccupancyMaxActiveBlocksPerMultiprocessor(int* b, const void*, int, size_t) { *b = 1; return hipSuccess; }
 inline hipError_t hipFuncSetAttribute(const void*, int, int) { return hipSuccess; }
 inline hipError_t hipEventCreate(hipEvent_t* e) { *e = nullptr; return hipSuccess; }

@@ -3,14 +3,16 @@
 tests/emu/ holds a CPU stand-in for the HIP runtime and device built-ins (one
 OS thread per lane, real barriers, wave exchanges, the 16x16x4 MFMA with the
 operand layout measured on gfx950).  The real planner (engine.cpp), the fused
-tile kernel (kernels_tile.hip) and the batch kernel (kernels_batch.hip) are
+tile kernel (kernels_tile.hip), the streaming kernels that run passes of one
+op (kernels_stream.hip) and the batch kernel (kernels_batch.hip) are
 compiled against it with clang++ -fsanitize=address, so every LDS and global
 access of those kernels is bounds-checked on the host, and their results are
 compared with the oracle:
 
   * fused passes of 3-5 target dense gates on 2^12- and 2^13-amplitude tiles
-    (MFMA and VALU forms; the VALU split mode for 2^13 FP64 tiles was wrong
-    before it was restructured into rounds of whole groups);
+    (MFMA form with the matrices staged in LDS, and VALU form; the VALU split
+    mode for 2^13 FP64 tiles was wrong before it was restructured into rounds
+    of whole groups);
   * a mixed random circuit on 2^13-amplitude tiles;
   * batched <H> and adjoint gradients, LDS-resident and streamed jobs, FP64 and
     FP32, against an adjoint computed with the oracle.
@@ -40,17 +42,19 @@ CODES = {"oneTargGate": 0, "manyCtrlOneTargGate": 1, "swapGate": 2, "manyTargGat
 def _build():
     BUILD.mkdir(exist_ok=True)
     exe = BUILD / "emu_drive"
-    sources = [CSRC / f for f in ("kernels_tile.hip", "kernels_batch.hip", "engine.cpp", "circuit.cpp", "batch.cpp",
-                                  "engine.hpp", "tile_ops.hpp", "common.hpp", "launch.hpp")]
+    sources = [CSRC / f for f in ("kernels_tile.hip", "kernels_stream.hip", "kernels_batch.hip", "engine.cpp",
+                                  "circuit.cpp", "batch.cpp", "engine.hpp", "tile_ops.hpp", "common.hpp", "launch.hpp")]
     sources += [EMU / "emu_rt.cpp", EMU / "emu_drive.cpp", EMU / "hip" / "hip_runtime.h"]
     if exe.exists() and all(s.stat().st_mtime < exe.stat().st_mtime for s in sources):
         return exe
-    # the tile kernel's dynamic LDS becomes a static array (workgroups run one at a time)
-    kt = (CSRC / "kernels_tile.hip").read_text().replace(
-        "extern __shared__ unsigned char smem_raw[];",
-        "static unsigned char smem_raw[262144] __attribute__((aligned(16)));")
-    (BUILD / "kernels_tile_emu.cpp").write_text(kt)
-    units = [(BUILD / "kernels_tile_emu.cpp", []), (CSRC / "kernels_batch.hip", ["-x", "c++"]),
+    # dynamic LDS becomes a static array (workgroups run one at a time)
+    for name in ("kernels_tile", "kernels_stream"):
+        src = (CSRC / f"{name}.hip").read_text().replace(
+            "extern __shared__ unsigned char smem_raw[];",
+            "static unsigned char smem_raw[262144] __attribute__((aligned(16)));")
+        (BUILD / f"{name}_emu.cpp").write_text(src)
+    units = [(BUILD / "kernels_tile_emu.cpp", []), (BUILD / "kernels_stream_emu.cpp", []),
+             (CSRC / "kernels_batch.hip", ["-x", "c++"]),
              (CSRC / "engine.cpp", []), (CSRC / "circuit.cpp", []), (CSRC / "batch.cpp", []),
              (EMU / "emu_rt.cpp", []), (EMU / "emu_drive.cpp", [])]
     procs, objs = [], []
@@ -151,12 +155,34 @@ def _circuit_file(path, orc, n, ops, rng):
 @pytest.mark.parametrize("bits", ["12", "13"])
 @pytest.mark.parametrize("mfma", ["1", "0"])
 def test_dense_passes_on_cpu(drive, orc, tmp_path, bits, mfma):
+    # passes full of dense gates hold more than the 16 KiB of MFMA matrices a
+    # workgroup keeps in LDS, so they take the VALU form
     rng = np.random.default_rng(31)
     n = 14
     path = tmp_path / "dense.bin"
     _circuit_file(path, orc, n, [_random_op(rng, n, dense_only=True) for _ in range(24)], rng)
     out = _run(drive, ["circuit", path], {"FSA_TILE_BITS": bits, "FSA_MFMA": mfma})
     assert out["rel_err"] <= 1e-12, out
+
+
+@pytest.mark.parametrize("bits", ["12", "13"])
+@pytest.mark.parametrize("ks", [(3,), (4,), (5,), (4, 4, 3)])
+def test_mfma_pass_with_matrices_in_lds_on_cpu(drive, orc, tmp_path, bits, ks):
+    # one pass: dense gates within the 16 KiB of LDS for MFMA matrices, among
+    # one-qubit gates, so it runs on MFMA with the matrices staged in LDS
+    rng = np.random.default_rng(33 + sum(ks))
+    n = 14
+    hi = [int(x) for x in rng.permutation(np.arange(4, n))]
+    ops = [("oneTargGate", (int(rng.integers(n)), _unitary(rng, 2))) for _ in range(6)]
+    for i, k in enumerate(ks):
+        q = [int(x) for x in rng.permutation(hi[:8])[:k - 1]] + [i]     # within 9 tile qubits
+        ops.append(("manyTargGate", (q, _unitary(rng, 1 << k))))
+        ops.append(("oneTargGate", (q[0], _unitary(rng, 2))))
+    path = tmp_path / "mfma.bin"
+    _circuit_file(path, orc, n, ops, rng)
+    out = _run(drive, ["circuit", path], {"FSA_TILE_BITS": bits})
+    assert out["rel_err"] <= 1e-12, out
+    assert out.get("tile_pass_mfma", 0) == 1 and out["passes"] == 1, out
 
 
 def test_mixed_circuit_on_13_bit_tiles_on_cpu(drive, orc, tmp_path):
