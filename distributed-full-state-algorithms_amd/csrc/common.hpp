@@ -117,9 +117,11 @@ static_assert(sizeof(TileOp) % 8 == 0, "TileOp must stay 8-byte aligned");
 
 constexpr uint32_t NO_PARAM = 0xffffffffu;
 constexpr uint32_t MAT_UNPLACED = 0xffffffffu;  // MFMA-form MATK item not yet given an LDS offset
-// LDS bytes per workgroup for MFMA matrices: two 64 KiB tiles plus two such
-// areas fill the 160 KiB of LDS of a CU
-constexpr uint32_t TILE_MAT_LDS_BYTES = 16384;
+// LDS bytes per workgroup for MFMA matrices: what a CU's 160 KiB of LDS leaves
+// next to two 64 KiB tiles (16 KiB each) or one 128 KiB tile (32 KiB)
+constexpr uint32_t tile_mat_lds_bytes(size_t tileBytes) {
+    return tileBytes > 65536 ? (uint32_t)(163840 - tileBytes) : 16384u;
+}
 constexpr uint32_t DIAG_RUN_MAX = 32;      // items per OP_DIAG_RUN
 constexpr uint32_t DIAG_RUN_GRADS = 6;     // gradient items per OP_DIAG_RUN
 constexpr int TILE_MAX_BITS = 14;

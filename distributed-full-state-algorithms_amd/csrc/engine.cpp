@@ -665,7 +665,7 @@ void execute_impl(State& s, const std::vector<HOp>& ops, const std::vector<Stage
         }
         stageMfma = stageMfma && anyMfma;
         // MFMA matrices sit in LDS (2 * 4^k reals each) when they fit in its area
-        const bool matsInLds = stageMfma && mfmaBytes <= TILE_MAT_LDS_BYTES;
+        const bool matsInLds = stageMfma && mfmaBytes <= tile_mat_lds_bytes(sizeof(cx<R>) << m);
         for (int oi : st.ops) {
             const HOp& op = ops[oi];
             if (useSweeps && !denseStage && m >= W && sweepable(op, W)) {
@@ -712,20 +712,12 @@ void execute_impl(State& s, const std::vector<HOp>& ops, const std::vector<Stage
                 TileOp& it = items[i];
                 if (it.kind != OP_MATK || it.lmask2 != 1u) continue;
                 const uint32_t need = (uint32_t)(2u * sizeof(R)) << (2 * it.k);
-                if (used + need > TILE_MAT_LDS_BYTES)
+                if (used + need > tile_mat_lds_bytes(sizeof(cx<R>) << m))
                     fail(FSA_ERR_INTERNAL, "MFMA matrices of a tile pass exceed its LDS budget");
                 it.lmask = used / (uint32_t)sizeof(R);
                 used += need;
             }
             matLds[si] = used;
-        }
-        // fuse the first / last sweep with the HBM load / store on full tiles when
-        // its bits leave the 3 lowest tile bits to the lanes (8 consecutive amplitudes)
-        auto fusable = [&](const TileOp& it) { return it.kind == OP_SWEEP && m == FULL && it.p[0] >= 3; };
-        if (count[si] > 0) {
-            if (fusable(items[first[si]])) flags[si] |= 1u;
-            if (fusable(items[first[si] + count[si] - 1])) flags[si] |= 2u;
-            if (count[si] == 1 && flags[si] != 3u) flags[si] = 0;
         }
         // kernel variant: 1 for dense gates / two-qubit depolarising, 2 for MFMA
         // dense gates with matrices in LDS, 3 with matrices in HBM; bit 4 marks sweeps

@@ -299,19 +299,17 @@ __device__ __forceinline__ uint64_t tile_scatter(uint32_t l, const TileGeom& g) 
     return r;
 }
 
+// s_waitcnt operand for vmcnt(0) with expcnt and lgkmcnt left at their maxima
+// (gfx9 encoding: vmcnt bits 3:0 and 15:14, expcnt 6:4, lgkmcnt 11:8)
+constexpr int WAIT_VMCNT0 = 0x0F70;
+
 // Persistent, software-pipelined pass.  A workgroup walks tiles blockIdx.x,
 // blockIdx.x + gridDim.x, ...; while the items of tile t run, the loads of the
 // next tile (into registers) and the write-back of the previous one are in
 // flight.  Barriers on gfx950 wait only for LDS traffic, and the items issue
 // no vector-memory loads (MFMA matrices are staged in LDS), so nothing waits
-// for that traffic before the items end.  FIN: the first item is a
-// sweep fused with the HBM load (registers in that sweep's layout); FOUT: the
-// last item is a sweep fused with the HBM store.  A pass made of a single
-// fused sweep never touches LDS.  ROWS = tile size / TB for row-layout loads.
-// s_waitcnt operand for vmcnt(0) with expcnt and lgkmcnt left at their maxima
-// (gfx9 encoding: vmcnt bits 3:0 and 15:14, expcnt 6:4, lgkmcnt 11:8)
-constexpr int WAIT_VMCNT0 = 0x0F70;
-
+// for that traffic before the items end.  SW: the pass has register sweeps
+// (OP_SWEEP items, one LDS round trip for a run of ops).  ROWS = tile size / TB.
 template <typename R, int TB, int ROWS, int W, int VAR, bool SW>
 __global__ __launch_bounds__(TB, (sizeof(R) == 8 && ROWS >= 16) ? 2 : 4) void k_tile(cx<R>* __restrict__ a, TileGeom geom,
                                                 const TileOp* __restrict__ items, unsigned numItems,
@@ -321,13 +319,11 @@ __global__ __launch_bounds__(TB, (sizeof(R) == 8 && ROWS >= 16) ? 2 : 4) void k_
     cx<R>* s = reinterpret_cast<cx<R>*>(smem_raw);
     constexpr int LOGTB = TB == 1024 ? 10 : TB == 512 ? 9 : TB == 256 ? 8 : 7;
     constexpr int LOGROWS = ROWS >= 16 ? 4 : ROWS >= 8 ? 3 : ROWS >= 4 ? 2 : ROWS >= 2 ? 1 : 0;
-    constexpr int G = 1 << W;
-    constexpr int NB = ROWS > G ? ROWS : G;
     const uint32_t m = geom.m;
     const uint32_t T = 1u << m;
     const uint32_t tid = threadIdx.x;
     const bool owner = tid < T;
-    const bool fin = SW && (flags & 1u) != 0, fout = SW && (flags & 2u) != 0;
+    (void)flags;
 
     // Addresses are ORs of per-bit values selected by compile-time register
     // indices, so only the per-bit values stay live.
@@ -337,39 +333,19 @@ __global__ __launch_bounds__(TB, (sizeof(R) == 8 && ROWS >= 16) ? 2 : 4) void k_
     uint64_t rowBit[LOGROWS > 0 ? LOGROWS : 1];
 #pragma unroll
     for (int j = 0; j < LOGROWS; j++) rowBit[j] = (uint32_t)(LOGTB + j) < m ? 1ULL << geom.q[LOGTB + j] : 0ULL;
-    // layout of the first sweep when it is fused with the load
-    uint64_t gIn[W];
-    uint64_t scIn = 0;
-    if (fin) {
-        uint32_t mi = 0;
-#pragma unroll
-        for (int j = 0; j < W; j++) { mi |= 1u << items[0].p[j]; gIn[j] = 1ULL << geom.q[items[0].p[j]]; }
-        scIn = tile_scatter(ins0_mask32(tid, mi), geom);
-    } else {
-#pragma unroll
-        for (int j = 0; j < W; j++) gIn[j] = 0;
-    }
 
 #define FSA_ROWADDR(r) ([&]() { uint64_t x_ = scLo; _Pragma("unroll") for (int j_ = 0; j_ < LOGROWS; j_++) if (((r) >> j_) & 1) x_ |= rowBit[j_]; return x_; }())
-#define FSA_SWADDR(i, g) ([&]() { uint64_t x_ = 0; _Pragma("unroll") for (int j_ = 0; j_ < W; j_++) if (((i) >> j_) & 1) x_ |= g[j_]; return x_; }())
 #define FSA_PREFETCH(t)                                                                                 \
     do {                                                                                                \
         const uint64_t pb_ = ins0_mask((t), geom.tileMask);                                             \
-        if (fin) {                                                                                      \
-            _Pragma("unroll") for (int i = 0; i < G; i++) {                                             \
-                cx<R> v_ = a[pb_ | scIn | FSA_SWADDR(i, gIn)]; bx[i] = v_.x; by[i] = v_.y;              \
-            }                                                                                           \
-        } else {                                                                                        \
-            _Pragma("unroll") for (int r = 0; r < ROWS; r++) {                                          \
-                cx<R> v_ = a[pb_ | FSA_ROWADDR(r)]; bx[r] = v_.x; by[r] = v_.y;                         \
-            }                                                                                           \
+        _Pragma("unroll") for (int r = 0; r < ROWS; r++) {                                              \
+            cx<R> v_ = a[pb_ | FSA_ROWADDR(r)]; bx[r] = v_.x; by[r] = v_.y;                             \
         }                                                                                               \
     } while (0)
 
     // Threads beyond a small tile load valid addresses of it (repeats) and
-    // neither store nor write LDS.  Sweep passes prefetch unconditionally: the
-    // last prefetch of a workgroup re-reads its current tile.
-    R bx[NB], by[NB];
+    // neither store nor write LDS.
+    R bx[ROWS], by[ROWS];
     uint64_t tile = blockIdx.x;
     if (tile >= numTiles) return;
     // MFMA matrices with an LDS offset are copied once, behind the tile, in the
@@ -389,29 +365,15 @@ __global__ __launch_bounds__(TB, (sizeof(R) == 8 && ROWS >= 16) ? 2 : 4) void k_
         __syncthreads();
     }
     FSA_PREFETCH(tile);
-    uint64_t next = tile + gridDim.x;
-    if constexpr (!SW) {
-        if (owner) {
+    if (owner) {
 #pragma unroll
-            for (int r = 0; r < ROWS; r++) s[((uint32_t)r << LOGTB) | tid] = mk<R>(bx[r], by[r]);
-        }
-        __syncthreads();
-        if (next < numTiles) FSA_PREFETCH(next);
+        for (int r = 0; r < ROWS; r++) s[((uint32_t)r << LOGTB) | tid] = mk<R>(bx[r], by[r]);
     }
+    __syncthreads();
+    uint64_t next = tile + gridDim.x;
+    if (next < numTiles) FSA_PREFETCH(next);
     for (;;) {
         const uint64_t base = ins0_mask(tile, geom.tileMask);
-        // sweep passes: the prefetch target, the current tile when it is the last
-        const uint64_t nx = next < numTiles ? next : tile;
-        if constexpr (SW) {
-            if (!fin) {
-                if (owner) {
-#pragma unroll
-                    for (int r = 0; r < ROWS; r++) s[((uint32_t)r << LOGTB) | tid] = mk<R>(bx[r], by[r]);
-                }
-                __syncthreads();
-                FSA_PREFETCH(nx);
-            }
-        }
         for (unsigned it = 0; it < numItems; it++) {
             const TileOp& op = items[it];
             if constexpr (VAR < 2) {   // the host builds no diagonal runs in MFMA passes
@@ -426,87 +388,56 @@ __global__ __launch_bounds__(TB, (sizeof(R) == 8 && ROWS >= 16) ? 2 : 4) void k_
                     continue;
                 }
             }
-            if (!SW || op.kind != OP_SWEEP) {
-                tile_apply<R, TB, VAR, (ROWS * TB >= 8192 ? ROWS * TB / 4096 : 1)>(s, T, base, op, mats, lmat);
-                __syncthreads();
-                continue;
-            }
-            // one code path for every sweep: source = prefetch registers (fused
-            // load) or LDS, destination = HBM (fused store) or LDS
-            const bool fromRegs = fin && it == 0;
-            const bool toHbm = fout && it == numItems - 1;
-            uint32_t smask = 0;
-#pragma unroll
-            for (int j = 0; j < W; j++) smask |= 1u << op.p[j];
-            for (uint32_t g = tid; g < (T >> W); g += TB) {
-                const uint32_t gb = ins0_mask32(g, smask);
-                cx<R> v[G];
-                if (fromRegs) {
-#pragma unroll
-                    for (int i = 0; i < G; i++) v[i] = mk<R>(bx[i], by[i]);
-                } else {
-#pragma unroll
-                    for (int i = 0; i < G; i++) v[i] = s[gb | sw_dep<W>(i, op.p)];
-                }
-                sweep_ops<R, W>(v, op, micro, gb, base, mats);
-                if (toHbm) {
-                    const uint64_t sc = base | tile_scatter(gb, geom);
-#pragma unroll
-                    for (int i = 0; i < G; i++) {
-                        uint64_t x = sc;
-#pragma unroll
-                        for (int j = 0; j < W; j++) if ((i >> j) & 1) x |= 1ULL << geom.q[op.p[j]];
-                        a[x] = v[i];
-                    }
-                } else {
-#pragma unroll
-                    for (int i = 0; i < G; i++) s[gb | sw_dep<W>(i, op.p)] = v[i];
+            if constexpr (SW) {
+                if (op.kind == OP_SWEEP) {
+                    sweep_lds<R, TB, W>(s, T, op, micro, base, mats);
+                    __syncthreads();
+                    continue;
                 }
             }
-            if (fromRegs) FSA_PREFETCH(nx);
-            if (!toHbm) __syncthreads();
-        }
-        if constexpr (SW) {
-            if (!fout && owner) {
-#pragma unroll
-                for (int r = 0; r < ROWS; r++) a[base | FSA_ROWADDR(r)] = s[((uint32_t)r << LOGTB) | tid];
-            }
+            tile_apply<R, TB, VAR, (ROWS >= 16 ? ROWS / 8 : 1)>(s, T, base, op, mats, lmat);
             __syncthreads();
-        } else {
-            // The write-back of this tile and the loads of the tile after the next
-            // are issued together, so both complete while the next tile's items
-            // run.  (Vector memory operations complete in order: stores issued
-            // just before the next tile is moved into LDS would be waited for
-            // there, leaving their latency exposed on every tile.)
-            // Each thread moves its own LDS slots out (to HBM) and the next
-            // tile in (from its registers), so no barrier separates the two;
-            // the wait below makes the next tile's loads land before the
-            // first store of this tile is issued.
-            __builtin_amdgcn_s_waitcnt(WAIT_VMCNT0);
-            const bool more = next < numTiles;
-            if (owner && more) {
-#pragma unroll
-                for (int r = 0; r < ROWS; r++) {
-                    const uint32_t l = ((uint32_t)r << LOGTB) | tid;
-                    const R vx = s[l].x, vy = s[l].y;
-                    s[l] = mk<R>(bx[r], by[r]);
-                    a[base | FSA_ROWADDR(r)] = mk<R>(vx, vy);
-                }
-            } else if (owner) {
-#pragma unroll
-                for (int r = 0; r < ROWS; r++) a[base | FSA_ROWADDR(r)] = s[((uint32_t)r << LOGTB) | tid];
-            }
-            if (!more) break;
-            if (next + gridDim.x < numTiles) FSA_PREFETCH(next + gridDim.x);
-            __syncthreads();   // the next tile is in LDS
         }
-        if (next >= numTiles) break;
+        // The write-back of this tile and the loads of the tile after the next
+        // are issued together, so both complete while the next tile's items
+        // run.  (Vector memory operations complete in order: stores issued
+        // just before the next tile is moved into LDS would be waited for
+        // there, leaving their latency exposed on every tile.)
+        // Each thread moves its own LDS slots out (to HBM) and the next tile in
+        // (from its registers), so no barrier separates the two; the wait below
+        // makes the next tile's loads land before the first store of this tile
+        // is issued.
+        __builtin_amdgcn_s_waitcnt(WAIT_VMCNT0);
+        const bool more = next < numTiles;
+        if (owner && more) {
+#pragma unroll
+            for (int r = 0; r < ROWS; r++) {
+                const uint32_t l = ((uint32_t)r << LOGTB) | tid;
+                const R vx = s[l].x, vy = s[l].y;
+                s[l] = mk<R>(bx[r], by[r]);
+                a[base | FSA_ROWADDR(r)] = mk<R>(vx, vy);
+            }
+        } else if (owner) {
+#pragma unroll
+            for (int r = 0; r < ROWS; r++) a[base | FSA_ROWADDR(r)] = s[((uint32_t)r << LOGTB) | tid];
+        }
+        if (!more) break;
+        if (next + gridDim.x < numTiles) FSA_PREFETCH(next + gridDim.x);
+        __syncthreads();   // the next tile is in LDS
         tile = next;
         next += gridDim.x;
     }
 #undef FSA_ROWADDR
-#undef FSA_SWADDR
 #undef FSA_PREFETCH
+}
+
+// kernel of a pass: TB threads, ROWS = tile / TB amplitudes per thread
+template <typename R, int TB, int RW>
+auto pick_tile(uint32_t var, bool sw) {
+    constexpr int W = sweep_bits<R>();
+    return var == 3 ? k_tile<R, TB, RW, W, 3, false> : var == 2 ? k_tile<R, TB, RW, W, 2, false>
+         : var == 1 ? k_tile<R, TB, RW, W, 1, false>
+         : sw ? k_tile<R, TB, RW, W, 0, true> : k_tile<R, TB, RW, W, 0, false>;
 }
 
 }  // namespace
@@ -514,8 +445,6 @@ __global__ __launch_bounds__(TB, (sizeof(R) == 8 && ROWS >= 16) ? 2 : 4) void k_
 template <typename R>
 void launch_tile(hipStream_t s, cx<R>* a, unsigned nq, const TileGeom& geom, const TileOp* ditems,
                  unsigned numItems, const TileOp* dmicro, const cx<R>* dmats, uint32_t flags, uint32_t matLds) {
-    constexpr int TB = 512;
-    constexpr int W = sweep_bits<R>();
     static int numCU = 0;
     if (numCU == 0) {
         int dev = 0;
@@ -525,28 +454,33 @@ void launch_tile(hipStream_t s, cx<R>* a, unsigned nq, const TileGeom& geom, con
     }
     uint64_t numTiles = 1ULL << (nq - geom.m);
     const uint32_t var = (flags >> 2) & 3u;
-    if (matLds > (var == 2 ? TILE_MAT_LDS_BYTES : 0u))
+    const size_t tileBytes = sizeof(cx<R>) << geom.m;
+    if (matLds > (var == 2 ? tile_mat_lds_bytes(tileBytes) : 0u))
         fail(FSA_ERR_INTERNAL, "tile pass asks for more LDS for matrices than its variant has");
-    const size_t lds = (sizeof(cx<R>) << geom.m) + matLds;
-    // enough resident workgroups to fill every CU: two 64 KiB tiles (each with up
-    // to 16 KiB of MFMA matrices) per CU, one 128 KiB tile
-    const uint64_t perCU = lds > 80 * 1024 ? 1 : 2;
-    uint64_t grid = std::min<uint64_t>(numTiles, (uint64_t)numCU * perCU);
-    const uint32_t rows = geom.m > 9 ? 1u << (geom.m - 9) : 1u;
+    const size_t lds = tileBytes + matLds;
+    // 16 waves per CU either way: two workgroups of 512 threads for tiles of up
+    // to 64 KiB, one of 1024 threads for a 128 KiB tile
+    const bool big = tileBytes > 64 * 1024;
+    const uint32_t tb = big ? 1024u : 512u;
+    const uint64_t grid = std::min<uint64_t>(numTiles, (uint64_t)numCU * (big ? 1u : 2u));
     const bool sw = (flags & 16u) != 0;
-    auto* kern = k_tile<R, TB, 1, W, 0, false>;
-#define FSA_PICK(RW)                                                                              \
-    kern = var == 3 ? k_tile<R, TB, RW, W, 3, false> : var == 2 ? k_tile<R, TB, RW, W, 2, false>          \
-         : var == 1 ? k_tile<R, TB, RW, W, 1, false>                                                 \
-         : sw ? k_tile<R, TB, RW, W, 0, true> : k_tile<R, TB, RW, W, 0, false>;
-    switch (rows) {
-    case 1: FSA_PICK(1) break;
-    case 2: FSA_PICK(2) break;
-    case 4: FSA_PICK(4) break;
-    case 8: FSA_PICK(8) break;
-    default: FSA_PICK(16) break;
+    auto* kern = k_tile<R, 512, 1, sweep_bits<R>(), 0, false>;
+    if (big) {
+        if constexpr (sizeof(R) == 8) {
+            if (geom.m != 13) fail(FSA_ERR_INTERNAL, "tile of more than 64 KiB that is not 2^13 FP64 amplitudes");
+            kern = pick_tile<R, 1024, 8>(var, sw);
+        } else {
+            fail(FSA_ERR_INTERNAL, "FP32 tiles hold at most 64 KiB");
+        }
+    } else {
+        switch (geom.m > 9 ? 1u << (geom.m - 9) : 1u) {
+        case 1: kern = pick_tile<R, 512, 1>(var, sw); break;
+        case 2: kern = pick_tile<R, 512, 2>(var, sw); break;
+        case 4: kern = pick_tile<R, 512, 4>(var, sw); break;
+        case 8: kern = pick_tile<R, 512, 8>(var, sw); break;
+        default: kern = pick_tile<R, 512, 16>(var, sw); break;
+        }
     }
-#undef FSA_PICK
     if (lds > 64 * 1024) {
         // dynamic LDS beyond 64 KiB must be granted per kernel before the launch
         static std::mutex mu;
@@ -560,7 +494,7 @@ void launch_tile(hipStream_t s, cx<R>* a, unsigned nq, const TileGeom& geom, con
             g = lds;
         }
     }
-    hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(TB), lds, s, a, geom, ditems, numItems, dmicro, dmats,
+    hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(tb), lds, s, a, geom, ditems, numItems, dmicro, dmats,
                        numTiles, flags);
 }
 

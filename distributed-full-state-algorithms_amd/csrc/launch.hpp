@@ -37,9 +37,8 @@ template <typename R>
 void launch_scale(hipStream_t s, cx<R>* a, uint64_t num, R f);
 
 // ---- fused tile pass ----------------------------------------------------------
-// items: plain ops or OP_SWEEP items whose micro-ops live in dmicro; flags bit 0:
-// the first item is a sweep fused with the HBM load, bit 1: the last with the
-// store, bits 2-3: kernel variant (0 simple ops, 1 adds VALU dense gates and
+// items: plain ops or OP_SWEEP items whose micro-ops live in dmicro; flags
+// bits 2-3: kernel variant (0 simple ops, 1 adds VALU dense gates and
 // two-qubit depolarising, 2 MFMA dense gates with matrices in LDS, 3 with
 // matrices in HBM), bit 4: the pass has sweeps.
 // matLds: bytes of LDS after the tile for MFMA matrices (variant 2 only).
@@ -51,7 +50,7 @@ template <typename R> constexpr int tile_bits_max() { return sizeof(R) == 8 ? 12
 // largest tile a pass may use when asked for (FSA_TILE_BITS): 128 KiB of LDS
 template <typename R> constexpr int tile_bits_limit() { return sizeof(R) == 8 ? 13 : 13; }
 // register-sweep width: 2^W amplitudes per thread = full tile / 512 threads
-template <typename R> constexpr int sweep_bits() { return sizeof(R) == 8 ? 3 : 4; }
+template <typename R> constexpr int sweep_bits() { return sizeof(R) == 8 ? 2 : 4; }
 
 // ---- init / copy / conversion ------------------------------------------------
 template <typename R>
