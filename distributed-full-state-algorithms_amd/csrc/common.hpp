@@ -89,6 +89,7 @@ enum OpKind : uint32_t {
     OP_SWEEP = 11,   // tile pass item: a run of ops applied to register groups (see kernels_tile.hip)
     OP_GRAD = 12,    // batched adjoint: accumulate <lambda|P|phi> over the tile (see kernels_batch.hip)
     OP_DIAG_RUN = 13,// header of k consecutive diagonal items applied in one LDS sweep (tile_ops.hpp)
+    OP_DIAG4 = 14,   // diagonal factor f[(parity1 << 1) | parity2] (pass-end item applied at the write-back)
 };
 
 // One item of a fused tile pass, or one micro-op inside a sweep item.
@@ -105,7 +106,8 @@ struct TileOp {
     uint32_t k;           // MATK: number of targets; SWEEP: number of micro-ops
     uint32_t p[6];        // positions (targets in caller order for item-form MATK, ascending in micro form)
     uint32_t lmask;       // MAT1: control mask; PAULI: XY mask; PHASE: parity mask;
-                          // MATK (MFMA form): offset (reals) of the matrix in the pass's LDS area
+                          // MATK (MFMA form): offset (reals) of the matrix in the pass's LDS area;
+                          // DIAG4: tile-local mask of parity 1 (lmask2: of parity 2; gmask, gq: out-of-tile masks)
     uint32_t lmask2;      // PAULI: YZ mask
     uint64_t gmask;       // MAT1: out-of-tile control mask; PAULI/PHASE: out-of-tile parity mask
     uint64_t gq;          // micro form: the same masks restricted to tile bits outside the sweep
@@ -116,6 +118,7 @@ struct TileOp {
 static_assert(sizeof(TileOp) % 8 == 0, "TileOp must stay 8-byte aligned");
 
 constexpr uint32_t NO_PARAM = 0xffffffffu;
+constexpr size_t DRAIN_MAX = 1;            // diagonal items a tile pass applies on its way out of LDS
 constexpr uint32_t MAT_UNPLACED = 0xffffffffu;  // MFMA-form MATK item not yet given an LDS offset
 // LDS bytes per workgroup for MFMA matrices: what a CU's 160 KiB of LDS leaves
 // next to two 64 KiB tiles (16 KiB each) or one 128 KiB tile (32 KiB)

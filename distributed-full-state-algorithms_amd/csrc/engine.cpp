@@ -611,6 +611,46 @@ TileOp translate_micro(const HOp& op, const int* pos, const int* sl, std::vector
     return t;
 }
 
+// A phase gadget or dephasing channel as an OP_DIAG4 item: amplitude l of a
+// tile with base index b is multiplied by f[(p1 << 1) | p2], p1 = parity of
+// (l & lmask) ^ parity of (b & gmask), p2 likewise with lmask2 and gq.  The
+// factors are the ones the op's own tile item multiplies by (1 where it
+// leaves an amplitude unchanged), so both forms give the same results.
+TileOp translate_diag4(const HOp& op, const int* pos, uint64_t tileMask) {
+    TileOp t;
+    std::memset(&t, 0, sizeof(t));
+    t.kind = OP_DIAG4;
+    t.param = NO_PARAM;
+    auto split = [&](uint64_t gm, uint32_t& lm, uint64_t& out) {
+        lm = 0;
+        for (uint64_t m = gm & tileMask; m; m &= m - 1) lm |= 1u << pos[__builtin_ctzll(m)];
+        out = gm & ~tileMask;
+    };
+    auto set = [&](int i, cd v) { t.f[2 * i] = v.real(); t.f[2 * i + 1] = v.imag(); };
+    switch (op.kind) {
+    case OP_PHASE: {   // e^{+i theta} on even parity, e^{-i theta} on odd
+        split(op.pmask, t.lmask, t.gmask);
+        const cd e(std::cos(op.theta), std::sin(op.theta)), o(std::cos(op.theta), -std::sin(op.theta));
+        set(0, e); set(1, e); set(2, o); set(3, o);
+        break;
+    }
+    case OP_DEPHASE1:   // scale where the bits of q and q + N differ
+    case OP_DEPHASE2: { // ... where those of q1 / q1 + N or of q2 / q2 + N differ
+        split((1ULL << op.q[0]) | (1ULL << op.q[1]), t.lmask, t.gmask);
+        if (op.kind == OP_DEPHASE2) split((1ULL << op.q[2]) | (1ULL << op.q[3]), t.lmask2, t.gq);
+        const cd one(1, 0), fac(op.kind == OP_DEPHASE2 ? 1.0 + op.c[0] : op.c[0], 0);   // the tile items' factors
+        set(0, one);
+        set(1, op.kind == OP_DEPHASE2 ? fac : one);
+        set(2, fac);
+        set(3, fac);
+        break;
+    }
+    default:
+        fail(FSA_ERR_INTERNAL, "op has no diagonal pass-end form");
+    }
+    return t;
+}
+
 template <typename R>
 void execute_impl(State& s, const std::vector<HOp>& ops, const std::vector<Stage>& stages) {
     constexpr unsigned W = (unsigned)sweep_bits<R>();
@@ -666,7 +706,21 @@ void execute_impl(State& s, const std::vector<HOp>& ops, const std::vector<Stage
         stageMfma = stageMfma && anyMfma;
         // MFMA matrices sit in LDS (2 * 4^k reals each) when they fit in its area
         const bool matsInLds = stageMfma && mfmaBytes <= tile_mat_lds_bytes(sizeof(cx<R>) << m);
-        for (int oi : st.ops) {
+        // Diagonal ops that commute with every later op of the pass move to its
+        // end (order kept among them); the kernel applies those in registers as
+        // the tile leaves LDS, without an LDS round trip of their own.  An op
+        // commutes with a later op when the later one mixes none of its qubits.
+        std::vector<int> body, tail;
+        // (not in FP32 MFMA passes: their kernels have no registers left for it)
+        const bool drainDiag = env_flag("FSA_DRAIN_DIAG", true) && !(sizeof(R) == 4 && stageMfma);
+        for (size_t i = 0; i < st.ops.size(); i++) {
+            const HOp& d = ops[st.ops[i]];
+            bool drain = drainDiag && tail.size() < DRAIN_MAX &&
+                         (d.kind == OP_PHASE || d.kind == OP_DEPHASE1 || d.kind == OP_DEPHASE2);
+            for (size_t j = i + 1; drain && j < st.ops.size(); j++) drain = (ops[st.ops[j]].nd & d.touch) == 0;
+            (drain ? tail : body).push_back(st.ops[i]);
+        }
+        for (int oi : body) {
             const HOp& op = ops[oi];
             if (useSweeps && !denseStage && m >= W && sweepable(op, W)) {
                 uint32_t nd = tile_local(op.nd, pos);
@@ -703,6 +757,8 @@ void execute_impl(State& s, const std::vector<HOp>& ops, const std::vector<Stage
             items.push_back(it);
         }
         if (!stageMfma) fuse_diag_runs(items, first[si], false);
+        for (int oi : tail) items.push_back(translate_diag4(ops[oi], pos, g.tileMask));
+        flags[si] |= (uint32_t)tail.size() << 8;
         count[si] = items.size() - first[si];
         // every MFMA matrix of the pass goes to LDS (the left half of the real
         // form, 2 * 4^k reals)

@@ -323,7 +323,11 @@ __global__ __launch_bounds__(TB, (sizeof(R) == 8 && ROWS >= 16) ? 2 : 4) void k_
     const uint32_t T = 1u << m;
     const uint32_t tid = threadIdx.x;
     const bool owner = tid < T;
-    (void)flags;
+    // the last item (nDrain = 1) is an OP_DIAG4 applied at the write-back; FP32
+    // MFMA kernels (twice the accumulators per thread) have no registers for it
+    constexpr bool DRAIN = !(sizeof(R) == 4 && VAR >= 2);
+    const unsigned nDrain = DRAIN ? (flags >> 8) & 0xffu : 0u;
+    const unsigned nBody = numItems - nDrain;
 
     // Addresses are ORs of per-bit values selected by compile-time register
     // indices, so only the per-bit values stay live.
@@ -374,7 +378,7 @@ __global__ __launch_bounds__(TB, (sizeof(R) == 8 && ROWS >= 16) ? 2 : 4) void k_
     if (next < numTiles) FSA_PREFETCH(next);
     for (;;) {
         const uint64_t base = ins0_mask(tile, geom.tileMask);
-        for (unsigned it = 0; it < numItems; it++) {
+        for (unsigned it = 0; it < nBody; it++) {
             const TileOp& op = items[it];
             if constexpr (VAR < 2) {   // the host builds no diagonal runs in MFMA passes
                 if (op.kind == OP_DIAG_RUN) {
@@ -409,7 +413,7 @@ __global__ __launch_bounds__(TB, (sizeof(R) == 8 && ROWS >= 16) ? 2 : 4) void k_
         // is issued.
         __builtin_amdgcn_s_waitcnt(WAIT_VMCNT0);
         const bool more = next < numTiles;
-        if (owner && more) {
+        if (owner && nDrain == 0 && more) {
 #pragma unroll
             for (int r = 0; r < ROWS; r++) {
                 const uint32_t l = ((uint32_t)r << LOGTB) | tid;
@@ -417,9 +421,32 @@ __global__ __launch_bounds__(TB, (sizeof(R) == 8 && ROWS >= 16) ? 2 : 4) void k_
                 s[l] = mk<R>(bx[r], by[r]);
                 a[base | FSA_ROWADDR(r)] = mk<R>(vx, vy);
             }
-        } else if (owner) {
+        } else if (owner && nDrain == 0) {
 #pragma unroll
             for (int r = 0; r < ROWS; r++) a[base | FSA_ROWADDR(r)] = s[((uint32_t)r << LOGTB) | tid];
+        } else if (owner) {
+            // the pass-end OP_DIAG4 item: amplitude l is multiplied by
+            // f[(p1 << 1) | p2], p1 / p2 the parities under its two masks
+            const TileOp& dop = items[nBody];
+            const uint32_t m1 = dop.lmask, m2 = dop.lmask2;
+            const uint32_t b1 = par64(base & dop.gmask), b2 = par64(base & dop.gq);
+            R fx0 = (R)dop.f[0], fy0 = (R)dop.f[1], fx1 = (R)dop.f[2], fy1 = (R)dop.f[3];
+            R fx2 = (R)dop.f[4], fy2 = (R)dop.f[5], fx3 = (R)dop.f[6], fy3 = (R)dop.f[7];
+            // empty register constraints (no instruction): without them the
+            // compiler turns the per-amplitude selects among these loaded
+            // values into an indexed load from a private copy of the table
+            asm volatile("" : "+v"(fx0), "+v"(fy0), "+v"(fx1), "+v"(fy1));
+            asm volatile("" : "+v"(fx2), "+v"(fy2), "+v"(fx3), "+v"(fy3));
+#pragma unroll
+            for (int r = 0; r < ROWS; r++) {
+                const uint32_t l = ((uint32_t)r << LOGTB) | tid;
+                const R vx = s[l].x, vy = s[l].y;
+                if (more) s[l] = mk<R>(bx[r], by[r]);
+                const uint32_t p1 = par32(l & m1) ^ b1, p2 = par32(l & m2) ^ b2;
+                const R fx = p1 ? (p2 ? fx3 : fx2) : (p2 ? fx1 : fx0);
+                const R fy = p1 ? (p2 ? fy3 : fy2) : (p2 ? fy1 : fy0);
+                a[base | FSA_ROWADDR(r)] = cmul(mk<R>(vx, vy), mk<R>(fx, fy));
+            }
         }
         if (!more) break;
         if (next + gridDim.x < numTiles) FSA_PREFETCH(next + gridDim.x);

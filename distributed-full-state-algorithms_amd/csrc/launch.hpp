@@ -40,7 +40,9 @@ void launch_scale(hipStream_t s, cx<R>* a, uint64_t num, R f);
 // items: plain ops or OP_SWEEP items whose micro-ops live in dmicro; flags
 // bits 2-3: kernel variant (0 simple ops, 1 adds VALU dense gates and
 // two-qubit depolarising, 2 MFMA dense gates with matrices in LDS, 3 with
-// matrices in HBM), bit 4: the pass has sweeps.
+// matrices in HBM), bit 4: the pass has sweeps, bits 8-15: the number of
+// trailing OP_DIAG4 items (0 or 1) applied in registers as each tile is
+// written back.
 // matLds: bytes of LDS after the tile for MFMA matrices (variant 2 only).
 template <typename R>
 void launch_tile(hipStream_t s, cx<R>* a, unsigned nq, const TileGeom& geom, const TileOp* ditems,

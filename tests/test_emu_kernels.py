@@ -13,6 +13,8 @@ compared with the oracle:
     (MFMA form with the matrices staged in LDS, and VALU form; the VALU split
     mode for 2^13 FP64 tiles was wrong before it was restructured into rounds
     of whole groups);
+  * a pass whose last item is a phase gadget applied in registers as each
+    tile is written back;
   * a mixed random circuit on 2^13-amplitude tiles;
   * batched <H> and adjoint gradients, LDS-resident and streamed jobs, FP64 and
     FP32, against an adjoint computed with the oracle.
@@ -183,6 +185,21 @@ def test_mfma_pass_with_matrices_in_lds_on_cpu(drive, orc, tmp_path, bits, ks):
     out = _run(drive, ["circuit", path], {"FSA_TILE_BITS": bits})
     assert out["rel_err"] <= 1e-12, out
     assert out.get("tile_pass_mfma", 0) == 1 and out["passes"] == 1, out
+
+
+@pytest.mark.parametrize("drain", ["1", "0"])
+def test_phase_gadget_at_pass_end_on_cpu(drive, orc, tmp_path, drain):
+    # one pass whose phase gadgets commute with the ops after them: one of them
+    # is applied in registers as each tile is written back (FSA_DRAIN_DIAG=1)
+    rng = np.random.default_rng(35)
+    n = 15
+    ops = [("oneTargGate", (q, _unitary(rng, 2))) for q in (3, 9, 12, 14)]
+    ops += [("phaseGadget", ([0, 5, 13], 0.7)), ("oneTargGate", (7, _unitary(rng, 2))),
+            ("phaseGadget", ([2, 9, 14], -1.1)), ("oneTargGate", (11, _unitary(rng, 2)))]
+    path = tmp_path / "drain.bin"
+    _circuit_file(path, orc, n, ops, rng)
+    out = _run(drive, ["circuit", path], {"FSA_DRAIN_DIAG": drain})
+    assert out["rel_err"] <= 1e-12 and out["passes"] == 1, out
 
 
 def test_mixed_circuit_on_13_bit_tiles_on_cpu(drive, orc, tmp_path):
