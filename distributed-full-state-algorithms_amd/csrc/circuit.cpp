@@ -227,23 +227,40 @@ void adjoint_impl(Circuit& c, State& psi0, const double* coeffs, unsigned T, con
     // results: slot 0 = <phi|lambda>, slot 1 + p = <lambda|P_p|phi>
     const unsigned P = c.numParams;
 
-    // Reverse sweep, op by op: <lambda|P_k|phi> for each gadget, then U_k^dag
-    // on phi and on lambda.  (The tile-pass form of this sweep,
-    // build_reverse_plan + launch_rev_pass, is not used by this entry point.)
+    // Reverse sweep, op by op: for a gadget, <lambda|P_k|phi> and U_k^dag on phi
+    // and on lambda in one pass over both states (k_adjoint_step); any other op
+    // is undone on each state by its own kernel.  (The tile-pass form of this
+    // sweep, build_reverse_plan + launch_rev_pass, is not used by this entry point.)
     size_t red = reduce_workspace_doubles(phi.s.num());
     double* ws = phi.s.workspace(red + 2 * (P + 1) + 8);
     double* res = ws + red;
     launch_inner<R>(st, a_phi, a_lam, phi.s.num(), ws, res);
     for (size_t k = c.ops.size(); k-- > 0;) {
         const HOp& op = c.ops[k];
+        const HOp inv = inverse(op);
         if (op.param >= 0) {
-            const PauliTermDev term = op.kind == OP_PAULI ? PauliTermDev{op.mxy, op.myz, op.pw.real(), op.pw.imag()}
-                                                          : PauliTermDev{0, op.pmask, 1.0, 0.0};
-            launch_pauli_element<R>(st, a_lam, a_phi, phi.s.num(), term, ws, res + 2 * (1 + op.param));
+            AdjointStepDev d;
+            std::memset(&d, 0, sizeof(d));
+            if (op.kind == OP_PAULI) {
+                d.mxy = op.mxy; d.myz = op.myz;
+                d.fr = op.pw.real(); d.fi = op.pw.imag();
+                d.thr = inv.th.real(); d.thi = inv.th.imag();
+                d.fer = inv.fe.real(); d.fei = inv.fe.imag();
+            } else {
+                d.mxy = 0; d.myz = op.pmask;
+                d.fr = 1.0; d.fi = 0.0;
+                d.phase = 1;
+                d.e0r = std::cos(inv.theta); d.e0i = std::sin(inv.theta);
+                d.e1r = std::cos(inv.theta); d.e1i = -std::sin(inv.theta);
+            }
+            ProfScope prof("adjoint_step", st);
+            launch_adjoint_step<R>(st, a_phi, a_lam, n, d, ws, res + 2 * (1 + op.param));
+            hip_check(hipGetLastError(), "adjoint step launch");
+            continue;
         }
-        std::vector<HOp> inv{inverse(op)};
-        execute_now(phi.s, inv);
-        execute_now(lam.s, inv);
+        std::vector<HOp> invs{inv};
+        execute_now(phi.s, invs);
+        execute_now(lam.s, invs);
     }
     std::vector<double> host(2 * (P + 1));
     hip_check(hipMemcpyAsync(host.data(), res, host.size() * sizeof(double), hipMemcpyDeviceToHost, st), "hipMemcpyAsync");

@@ -135,6 +135,58 @@ __global__ __launch_bounds__(BLOCK) void k_pauli_element(const cx<R>* __restrict
     }
 }
 
+// One step of the adjoint reverse sweep for a parametrised gadget, in one pass
+// over phi and lambda: the term <lambda| f P |phi> (as k_pauli_element), then
+// U^dag on both states with the arithmetic of k_pauli / k_pauli_diag / k_phase.
+// Each pair (j, j ^ mxy) of both states is read once and written once.
+template <typename R>
+__global__ __launch_bounds__(BLOCK) void k_adjoint_step(cx<R>* __restrict__ phi, cx<R>* __restrict__ lam,
+                                                        uint64_t numIts, unsigned hb, AdjointStepDev st,
+                                                        double* __restrict__ part) {
+    const cx<R> th = mk<R>((R)st.thr, (R)st.thi), fe = mk<R>((R)st.fer, (R)st.fei);
+    const cx<R> fo = mk<R>(-fe.x, -fe.y);
+    const cx<R> e0 = mk<R>((R)st.e0r, (R)st.e0i), e1 = mk<R>((R)st.e1r, (R)st.e1i);
+    double vr = 0, vi = 0;
+    for (uint64_t k = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; k < numIts; k += (uint64_t)gridDim.x * BLOCK) {
+        if (st.mxy) {
+            const uint64_t j0 = ins0(k, hb), j1 = j0 ^ st.mxy;
+            const cx<R> p0 = phi[j0], p1 = phi[j1], l0 = lam[j0], l1 = lam[j1];
+            const bool o0 = par64(j0 & st.myz) != 0, o1 = par64(j1 & st.myz) != 0;
+            const double s0 = o0 ? -1.0 : 1.0, s1 = o1 ? -1.0 : 1.0;
+            // conj(lambda_j) (-1)^{par(j' & myz)} phi_j' for j = j0 and j = j1
+            vr += s1 * ((double)l0.x * (double)p1.x + (double)l0.y * (double)p1.y) +
+                  s0 * ((double)l1.x * (double)p0.x + (double)l1.y * (double)p0.y);
+            vi += s1 * ((double)l0.x * (double)p1.y - (double)l0.y * (double)p1.x) +
+                  s0 * ((double)l1.x * (double)p0.y - (double)l1.y * (double)p0.x);
+            const cx<R> f0 = o0 ? fo : fe, f1 = o1 ? fo : fe;
+            phi[j0] = cfma(cmul(th, p0), f1, p1);
+            phi[j1] = cfma(cmul(th, p1), f0, p0);
+            lam[j0] = cfma(cmul(th, l0), f1, l1);
+            lam[j1] = cfma(cmul(th, l1), f0, l0);
+        } else {
+            const cx<R> p = phi[k], l = lam[k];
+            const bool odd = par64(k & st.myz) != 0;
+            const double sg = odd ? -1.0 : 1.0;
+            vr += sg * ((double)l.x * (double)p.x + (double)l.y * (double)p.y);
+            vi += sg * ((double)l.x * (double)p.y - (double)l.y * (double)p.x);
+            if (st.phase) {
+                const cx<R> f = odd ? e1 : e0;
+                phi[k] = cmul(p, f);
+                lam[k] = cmul(l, f);
+            } else {
+                const cx<R> f = odd ? fo : fe;
+                phi[k] = cfma(cmul(th, p), f, p);
+                lam[k] = cfma(cmul(th, l), f, l);
+            }
+        }
+    }
+    block_sum2(vr, vi);
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = st.fr * vr - st.fi * vi;
+        part[2 * blockIdx.x + 1] = st.fr * vi + st.fi * vr;
+    }
+}
+
 // out[j] = sum_t f_t (-1)^{par(j1 & myz_t)} in[j1], j1 = j ^ mxy_t
 template <typename R>
 __global__ __launch_bounds__(BLOCK) void k_apply_pauli_sum(const cx<R>* __restrict__ in, cx<R>* __restrict__ out,
@@ -262,6 +314,16 @@ void launch_pauli_element(hipStream_t s, const cx<R>* a, const cx<R>* b, uint64_
     hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(BLOCK), 0, s, ws, nb, dout);
 }
 
+template <typename R>
+void launch_adjoint_step(hipStream_t s, cx<R>* phi, cx<R>* lam, unsigned nq, const AdjointStepDev& st, double* ws,
+                         double* dout) {
+    const uint64_t numIts = st.mxy ? 1ULL << (nq - 1) : 1ULL << nq;
+    const unsigned hb = st.mxy ? 63u - (unsigned)__builtin_clzll(st.mxy) : 0u;
+    unsigned nb = red_blocks(numIts);
+    hipLaunchKernelGGL(k_adjoint_step<R>, dim3(nb), dim3(BLOCK), 0, s, phi, lam, numIts, hb, st, ws);
+    hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(BLOCK), 0, s, ws, nb, dout);
+}
+
 // ws must hold 2 * numTerms * ceil(2^nbits / EXP_CHUNK) doubles
 template <typename R>
 void launch_dm_expec(hipStream_t s, const cx<R>* rho, unsigned N, const PauliTermDev* dterms,
@@ -340,6 +402,8 @@ void launch_f32_to_f64(hipStream_t s, const cx<float>* in, cx<double>* out, uint
 #define FSA_INSTANTIATE_RED(R)                                                                               \
     template void launch_norm2<R>(hipStream_t, const cx<R>*, uint64_t, double*, double*);                     \
     template void launch_inner<R>(hipStream_t, const cx<R>*, const cx<R>*, uint64_t, double*, double*);       \
+    template void launch_adjoint_step<R>(hipStream_t, cx<R>*, cx<R>*, unsigned, const AdjointStepDev&, double*,  \
+                                         double*);                                                              \
     template void launch_pauli_element<R>(hipStream_t, const cx<R>*, const cx<R>*, uint64_t, PauliTermDev,     \
                                           double*, double*);                                                  \
     template void launch_dm_expec<R>(hipStream_t, const cx<R>*, unsigned, const PauliTermDev*, unsigned,       \

@@ -74,6 +74,18 @@ void launch_inner(hipStream_t s, const cx<R>* a, const cx<R>* b, uint64_t num, d
 // sum over terms of coeff_t * (-i)^{nY_t} * sum_r sign * rho[r + 2^N (r ^ mxy_t)]
 struct PauliTermDev { uint64_t mxy, myz; double fr, fi; };   // factor f = coeff * phase
 uint64_t expec_workspace_doubles(unsigned nbits, unsigned numTerms);
+// one adjoint reverse step for a parametrised gadget (kernels_reduce.hip)
+struct AdjointStepDev {
+    uint64_t mxy, myz;            // Pauli masks (phase gadget: mxy = 0, myz = its parity mask)
+    double fr, fi;                // factor of the term <lambda| f P |phi> (i^{#Y}; 1 for phase gadgets)
+    double thr, thi, fer, fei;    // Pauli gadget U^dag: v_j <- th v_j + (+-fe) v_{j ^ mxy}
+    double e0r, e0i, e1r, e1i;    // phase gadget U^dag: v_j <- v_j e0 (even parity) or e1 (odd)
+    uint32_t phase;               // 1: phase gadget
+    uint32_t pad;
+};
+template <typename R>
+void launch_adjoint_step(hipStream_t s, cx<R>* phi, cx<R>* lam, unsigned nq, const AdjointStepDev& st, double* ws,
+                         double* dout);
 template <typename R>
 void launch_pauli_element(hipStream_t s, const cx<R>* a, const cx<R>* b, uint64_t num, PauliTermDev term,
                           double* ws, double* dout);
