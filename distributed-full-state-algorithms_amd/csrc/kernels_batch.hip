@@ -22,6 +22,7 @@
 #include "tile_ops.hpp"
 
 #include <algorithm>
+#include <type_traits>
 
 namespace fsa {
 
@@ -277,64 +278,96 @@ __global__ __launch_bounds__(TB) void k_batch(const BatchJob* __restrict__ jobs,
         cx<R>* gphi = arena + J.phi;
         cx<R>* glam = arena + J.lam;
         const uint64_t all = (J.n >= 64) ? ~0ULL : ((1ULL << J.n) - 1ULL);
+        // one streamed pass of stage s; REV: the reverse sweep (phi and lambda)
+        auto stream_pass = [&](auto revTag, uint32_t s, bool fromZero) {
+            constexpr bool REV = decltype(revTag)::value;
+            const BatchStage st = stages[s];
+            const uint32_t T = 1u << st.m;
+            const uint32_t K = T > TB ? T / TB : 1u;
+            const uint64_t outMask = all & ~st.tileMask;
+            const uint64_t numTiles = 1ULL << (J.n - st.m);
+            __syncthreads();
+            // every row gets an offset; rows past K alias in-tile addresses (pdep
+            // drops the bits above the tile), so the prefetch needs no row guard
+            if (tid < KMAX) offHi[tid] = pdep64((uint64_t)tid * TB, st.tileMask);
+            const uint64_t offLo = pdep64(tid, st.tileMask);
+            __syncthreads();
+            // Tiles of one pass are disjoint: while the items of tile t run in
+            // LDS, the loads of tile t + 1 (registers) and the write-back of tile
+            // t - 1 are in flight.  The loads are unconditional (a value merged
+            // from branches is copied, and so waited for, right after its load);
+            // the first forward pass, which starts from |0..0>, reads the arena
+            // and ignores what it read.  A tile here has T >= 2^10 >= TB
+            // amplitudes, K = T / TB <= KMAX per thread.
+            R px[KMAX], py[KMAX], qx[KMAX], qy[KMAX];
+            auto load = [&](uint64_t tt) {
+                const uint64_t b = pdep64(tt, outMask);
+#pragma unroll
+                for (uint32_t k = 0; k < KMAX; k++) {
+                    const cx<R> v = gphi[b | offLo | offHi[k]];
+                    px[k] = v.x;
+                    py[k] = v.y;
+                    if constexpr (REV) {
+                        const cx<R> w = glam[b | offLo | offHi[k]];
+                        qx[k] = w.x;
+                        qy[k] = w.y;
+                    }
+                }
+            };
+            // amplitude of row k at global index g: |0..0> on the first pass, else loaded
+            auto phiAt = [&](uint32_t k, uint64_t g) {
+                return fromZero ? mk<R>(g == 0 ? (R)1 : (R)0, (R)0) : mk<R>(px[k], py[k]);
+            };
+            load(0);
+#pragma unroll
+            for (uint32_t k = 0; k < KMAX; k++) {
+                if (k < K) {
+                    lds[tid + k * TB] = phiAt(k, offLo | offHi[k]);
+                    if constexpr (REV) lds[TMAX + tid + k * TB] = mk<R>(qx[k], qy[k]);
+                }
+            }
+            __syncthreads();
+            if (1 < numTiles) load(1);
+            for (uint64_t t = 0; t < numTiles; t++) {
+                const uint64_t base = pdep64(t, outMask);
+                run_items<R, TB, REV, false>(lds, lds + TMAX, T, base, items, st.item0, st.nitems, mats, params, gacc,
+                                             red, dtab, t == 0);
+                __syncthreads();   // the last item may be a gradient sum still reading other rows
+                // Each thread writes its own rows of tile t back and fills them
+                // with tile t + 1; the wait makes tile t + 1's loads land before the
+                // first store of tile t is issued (vector memory operations
+                // complete in order).
+                __builtin_amdgcn_s_waitcnt(WAIT_VMCNT0);
+                const bool more = t + 1 < numTiles;
+                const uint64_t nbase = more ? pdep64(t + 1, outMask) : 0;
+#pragma unroll
+                for (uint32_t k = 0; k < KMAX; k++) {
+                    if (k < K) {
+                        const uint32_t l = tid + k * TB;
+                        const uint64_t o = offLo | offHi[k];
+                        const R vx = lds[l].x, vy = lds[l].y;
+                        if (more) lds[l] = phiAt(k, nbase | o);
+                        gphi[base | o] = mk<R>(vx, vy);
+                        if constexpr (REV) {
+                            const R wx = lds[TMAX + l].x, wy = lds[TMAX + l].y;
+                            if (more) lds[TMAX + l] = mk<R>(qx[k], qy[k]);
+                            glam[base | o] = mk<R>(wx, wy);
+                        }
+                    }
+                }
+                if (!more) break;
+                if (t + 2 < numTiles) load(t + 2);
+                __syncthreads();   // tile t + 1 is in LDS
+            }
+            __syncthreads();   // the pass is in the arena before anything reads it
+            if constexpr (REV) flush_grads<TB>(st, items, gacc, res);
+        };
         for (int dir = 0; dir < 2; dir++) {
             const uint32_t s0 = dir == 0 ? J.fwd0 : J.rev0;
             const uint32_t ns = dir == 0 ? J.nfwd : J.nrev;
             for (uint32_t s = s0; s < s0 + ns; s++) {
-                const BatchStage st = stages[s];
-                const uint32_t T = 1u << st.m;
-                const uint32_t K = T > TB ? T / TB : 1u;
-                const uint64_t outMask = all & ~st.tileMask;
-                const uint64_t numTiles = 1ULL << (J.n - st.m);
-                __syncthreads();
-                if (tid < K) offHi[tid] = pdep64((uint64_t)tid * TB, st.tileMask);
-                const uint64_t offLo = pdep64(tid, st.tileMask);
-                __syncthreads();
-                const bool fromZero = dir == 0 && s == s0;   // the first pass starts from |0..0>
-                // Tiles of one pass are disjoint, so tile t + 1 is loaded into
-                // registers while tile t is worked on in LDS.  A tile here has
-                // T >= 2^10 >= TB amplitudes, K = T / TB <= KMAX per thread.
-                R px[KMAX], py[KMAX], qx[KMAX], qy[KMAX];
-#define FSA_BATCH_LOAD(tt)                                                                          \
-    do {                                                                                            \
-        const uint64_t b_ = pdep64((tt), outMask);                                                  \
-        _Pragma("unroll") for (uint32_t k = 0; k < KMAX; k++) {                                     \
-            if (k < K) {                                                                            \
-                const uint64_t g = b_ | offLo | offHi[k];                                           \
-                const cx<R> v = fromZero ? mk<R>(g == 0 ? (R)1 : (R)0, (R)0) : gphi[g];            \
-                px[k] = v.x; py[k] = v.y;                                                           \
-                if (dir == 1) { const cx<R> w = glam[g]; qx[k] = w.x; qy[k] = w.y; }               \
-            }                                                                                       \
-        }                                                                                           \
-    } while (0)
-                FSA_BATCH_LOAD(0);
-                for (uint64_t t = 0; t < numTiles; t++) {
-                    const uint64_t base = pdep64(t, outMask);
-#pragma unroll
-                    for (uint32_t k = 0; k < KMAX; k++) {
-                        if (k < K) {
-                            lds[tid + k * TB] = mk<R>(px[k], py[k]);
-                            if (dir == 1) lds[TMAX + tid + k * TB] = mk<R>(qx[k], qy[k]);
-                        }
-                    }
-                    __syncthreads();
-                    if (t + 1 < numTiles) FSA_BATCH_LOAD(t + 1);
-                    if (dir == 0)
-                        run_items<R, TB, false, false>(lds, lds + TMAX, T, base, items, st.item0, st.nitems, mats, params,
-                                                gacc, red, dtab, t == 0);
-                    else
-                        run_items<R, TB, true, false>(lds, lds + TMAX, T, base, items, st.item0, st.nitems, mats, params,
-                                               gacc, red, dtab, t == 0);
-                    for (uint32_t k = 0; k < K; k++) {
-                        const uint32_t l = tid + k * TB;
-                        const uint64_t g = base | offLo | offHi[k];
-                        gphi[g] = lds[l];
-                        if (dir == 1) glam[g] = lds[TMAX + l];
-                    }
-                    __syncthreads();
-                }
-#undef FSA_BATCH_LOAD
-                if (dir == 1) flush_grads<TB>(st, items, gacc, res);
+                if (dir == 0) stream_pass(std::false_type{}, s, s == s0);
+                else stream_pass(std::true_type{}, s, false);
             }
             if (dir == 0) {
                 // the forward pass is complete in the arena (this workgroup's own

@@ -299,10 +299,6 @@ __device__ __forceinline__ uint64_t tile_scatter(uint32_t l, const TileGeom& g) 
     return r;
 }
 
-// s_waitcnt operand for vmcnt(0) with expcnt and lgkmcnt left at their maxima
-// (gfx9 encoding: vmcnt bits 3:0 and 15:14, expcnt 6:4, lgkmcnt 11:8)
-constexpr int WAIT_VMCNT0 = 0x0F70;
-
 // Persistent, software-pipelined pass.  A workgroup walks tiles blockIdx.x,
 // blockIdx.x + gridDim.x, ...; while the items of tile t run, the loads of the
 // next tile (into registers) and the write-back of the previous one are in

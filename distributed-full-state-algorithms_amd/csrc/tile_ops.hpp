@@ -23,6 +23,10 @@ __device__ __forceinline__ uint32_t chan_bit(uint32_t l, uint64_t base, uint32_t
 template <typename R>
 __device__ __forceinline__ cx<R> ldc(const double* f, int i) { return mk<R>((R)f[2 * i], (R)f[2 * i + 1]); }
 
+// s_waitcnt operand for vmcnt(0) with expcnt and lgkmcnt left at their maxima
+// (gfx9 encoding: vmcnt bits 3:0 and 15:14, expcnt 6:4, lgkmcnt 11:8)
+constexpr int WAIT_VMCNT0 = 0x0F70;
+
 // dense k-target gate on LDS.  When the tile holds at least TB groups each
 // thread owns whole groups (in place).  Otherwise the 2^K outputs of a group
 // are split over TB / groups threads: every thread streams the group's inputs
