@@ -28,6 +28,20 @@ template <typename R>
 __host__ __device__ __forceinline__ cx<R> cadd(cx<R> a, cx<R> b) { return mk<R>(a.x + b.x, a.y + b.y); }
 template <typename R>
 __host__ __device__ __forceinline__ cx<R> cscale(cx<R> a, R s) { return mk<R>(a.x * s, a.y * s); }
+
+// Non-temporal amplitude load / store (streamed state traffic that is not
+// reused within the kernel: 2-3% faster than cached accesses on gfx950)
+template <typename R>
+__device__ __forceinline__ cx<R> ntl(const cx<R>* p) {
+    const R* q = reinterpret_cast<const R*>(p);
+    return mk<R>(__builtin_nontemporal_load(q), __builtin_nontemporal_load(q + 1));
+}
+template <typename R>
+__device__ __forceinline__ void nts(cx<R>* p, cx<R> v) {
+    R* q = reinterpret_cast<R*>(p);
+    __builtin_nontemporal_store(v.x, q);
+    __builtin_nontemporal_store(v.y, q + 1);
+}
 // acc + a*b
 template <typename R>
 __host__ __device__ __forceinline__ cx<R> cfma(cx<R> acc, cx<R> a, cx<R> b) {

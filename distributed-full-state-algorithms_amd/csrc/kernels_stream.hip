@@ -14,6 +14,23 @@ namespace {
 
 constexpr int ITEMS = 4;
 
+// Amplitude access, non-temporal (NT) when every 128-byte line a wave touches
+// is written whole by one store instruction; kernels whose mixed bits fall
+// inside a line write alternating elements per instruction and stay cached.
+template <bool NT, typename R>
+__device__ __forceinline__ cx<R> ld(const cx<R>* p) {
+    if constexpr (NT) return ntl(p);
+    else return *p;
+}
+template <bool NT, typename R>
+__device__ __forceinline__ void st(cx<R>* p, cx<R> v) {
+    if constexpr (NT) nts(p, v);
+    else *p = v;
+}
+// amplitude-index bits inside one 128-byte line
+template <typename R>
+constexpr uint64_t line_bits() { return 128 / sizeof(cx<R>) - 1; }
+
 inline unsigned blocks_for(uint64_t numItems, int itemsPerThread) {
     uint64_t per = (uint64_t)BLOCK * itemsPerThread;
     uint64_t b = (numItems + per - 1) / per;
@@ -22,7 +39,7 @@ inline unsigned blocks_for(uint64_t numItems, int itemsPerThread) {
 
 // local_statevector.hpp:14-51 (oneTargGate and manyCtrlOneTargGate): pair
 // (i0, i1 = i0 | 2^t) with every control bit set.  qmask = controls | target.
-template <typename R>
+template <typename R, bool NT>
 __global__ __launch_bounds__(BLOCK) void k_mat1(cx<R>* __restrict__ a, uint64_t numIts, unsigned t,
                                                 uint64_t qmask, uint64_t cmask, Mat2<R> g) {
     const uint64_t first = (uint64_t)blockIdx.x * (BLOCK * ITEMS) + threadIdx.x;
@@ -33,20 +50,20 @@ __global__ __launch_bounds__(BLOCK) void k_mat1(cx<R>* __restrict__ a, uint64_t 
     for (int p = 0; p < ITEMS; p++) {
         uint64_t j = first + (uint64_t)p * BLOCK;
         idx[p] = ins0_mask(j, qmask) | cmask;
-        if (j < numIts) { v0[p] = a[idx[p]]; v1[p] = a[idx[p] | tb]; }
+        if (j < numIts) { v0[p] = ld<NT>(a + idx[p]); v1[p] = ld<NT>(a + (idx[p] | tb)); }
     }
 #pragma unroll
     for (int p = 0; p < ITEMS; p++) {
         uint64_t j = first + (uint64_t)p * BLOCK;
         if (j < numIts) {
-            a[idx[p]] = cfma(cmul(g.m[0], v0[p]), g.m[1], v1[p]);
-            a[idx[p] | tb] = cfma(cmul(g.m[2], v0[p]), g.m[3], v1[p]);
+            st<NT>(a + idx[p], cfma(cmul(g.m[0], v0[p]), g.m[1], v1[p]));
+            st<NT>(a + (idx[p] | tb), cfma(cmul(g.m[2], v0[p]), g.m[3], v1[p]));
         }
     }
 }
 
 // local_statevector.hpp:54-69: swap amplitudes |..1..0..> <-> |..0..1..>
-template <typename R>
+template <typename R, bool NT>
 __global__ __launch_bounds__(BLOCK) void k_swap(cx<R>* __restrict__ a, uint64_t numIts, uint64_t b1,
                                                 uint64_t b2) {
     const uint64_t first = (uint64_t)blockIdx.x * (BLOCK * ITEMS) + threadIdx.x;
@@ -58,21 +75,21 @@ __global__ __launch_bounds__(BLOCK) void k_swap(cx<R>* __restrict__ a, uint64_t 
         uint64_t k = first + (uint64_t)p * BLOCK;
         idx[p] = ins0_mask(k, m);
         if (k < numIts) {
-            cx<R> u = a[idx[p] | b1], v = a[idx[p] | b2];
+            cx<R> u = ld<NT>(a + (idx[p] | b1)), v = ld<NT>(a + (idx[p] | b2));
             ax[p] = u.x; ay[p] = u.y; bx[p] = v.x; by[p] = v.y;
         }
     }
 #pragma unroll
     for (int p = 0; p < ITEMS; p++) {
         uint64_t k = first + (uint64_t)p * BLOCK;
-        if (k < numIts) { a[idx[p] | b1] = mk<R>(bx[p], by[p]); a[idx[p] | b2] = mk<R>(ax[p], ay[p]); }
+        if (k < numIts) { st<NT>(a + (idx[p] | b1), mk<R>(bx[p], by[p])); st<NT>(a + (idx[p] | b2), mk<R>(ax[p], ay[p])); }
     }
 }
 
 // local_statevector.hpp:102-135 (with the setBit defect corrected): pairs
 // (j0, j1 = j0 ^ mxy).  new[j0] = this*a0 + f(j1)*a1, new[j1] = this*a1 + f(j0)*a0,
 // f(j) = evenFac * (-1)^{par(j & myz)}, evenFac = other * i^{#Y}.
-template <typename R>
+template <typename R, bool NT>
 __global__ __launch_bounds__(BLOCK) void k_pauli(cx<R>* __restrict__ a, uint64_t numIts, unsigned hb,
                                                  uint64_t mxy, uint64_t myz, cx<R> th, cx<R> fe) {
     const uint64_t first = (uint64_t)blockIdx.x * (BLOCK * ITEMS) + threadIdx.x;
@@ -83,7 +100,7 @@ __global__ __launch_bounds__(BLOCK) void k_pauli(cx<R>* __restrict__ a, uint64_t
     for (int p = 0; p < ITEMS; p++) {
         uint64_t k = first + (uint64_t)p * BLOCK;
         idx[p] = ins0(k, hb);
-        if (k < numIts) { v0[p] = a[idx[p]]; v1[p] = a[idx[p] ^ mxy]; }
+        if (k < numIts) { v0[p] = ld<NT>(a + idx[p]); v1[p] = ld<NT>(a + (idx[p] ^ mxy)); }
     }
 #pragma unroll
     for (int p = 0; p < ITEMS; p++) {
@@ -92,8 +109,8 @@ __global__ __launch_bounds__(BLOCK) void k_pauli(cx<R>* __restrict__ a, uint64_t
             uint64_t j0 = idx[p], j1 = j0 ^ mxy;
             cx<R> f0 = par64(j0 & myz) ? fo : fe;
             cx<R> f1 = par64(j1 & myz) ? fo : fe;
-            a[j0] = cfma(cmul(th, v0[p]), f1, v1[p]);
-            a[j1] = cfma(cmul(th, v1[p]), f0, v0[p]);
+            st<NT>(a + j0, cfma(cmul(th, v0[p]), f1, v1[p]));
+            st<NT>(a + j1, cfma(cmul(th, v1[p]), f0, v0[p]));
         }
     }
 }
@@ -108,14 +125,14 @@ __global__ __launch_bounds__(BLOCK) void k_pauli_diag(cx<R>* __restrict__ a, uin
 #pragma unroll
     for (int p = 0; p < ITEMS; p++) {
         uint64_t j = first + (uint64_t)p * BLOCK;
-        if (j < num) v[p] = a[j];
+        if (j < num) v[p] = ntl(a + j);
     }
 #pragma unroll
     for (int p = 0; p < ITEMS; p++) {
         uint64_t j = first + (uint64_t)p * BLOCK;
         if (j < num) {
             cx<R> f = par64(j & myz) ? fo : fe;
-            a[j] = cfma(cmul(th, v[p]), f, v[p]);
+            nts(a + j, cfma(cmul(th, v[p]), f, v[p]));
         }
     }
 }
@@ -129,12 +146,12 @@ __global__ __launch_bounds__(BLOCK) void k_phase(cx<R>* __restrict__ a, uint64_t
 #pragma unroll
     for (int p = 0; p < ITEMS; p++) {
         uint64_t j = first + (uint64_t)p * BLOCK;
-        if (j < num) v[p] = a[j];
+        if (j < num) v[p] = ntl(a + j);
     }
 #pragma unroll
     for (int p = 0; p < ITEMS; p++) {
         uint64_t j = first + (uint64_t)p * BLOCK;
-        if (j < num) a[j] = cmul(v[p], par64(j & mask) ? f1 : f0);
+        if (j < num) nts(a + j, cmul(v[p], par64(j & mask) ? f1 : f0));
     }
 }
 
@@ -144,13 +161,13 @@ __global__ __launch_bounds__(BLOCK) void k_scale(cx<R>* __restrict__ a, uint64_t
 #pragma unroll
     for (int p = 0; p < ITEMS; p++) {
         uint64_t j = first + (uint64_t)p * BLOCK;
-        if (j < num) a[j] = cscale(a[j], f);
+        if (j < num) nts(a + j, cscale(ntl(a + j), f));
     }
 }
 
 // local_statevector.hpp:72-99 (setBit defect corrected): one thread per group
 // of 2^K amplitudes, matrix staged in LDS and read with wave-uniform addresses.
-template <typename R, int K>
+template <typename R, int K, bool NT>
 __global__ __launch_bounds__(BLOCK) void k_matk_reg(cx<R>* __restrict__ a, uint64_t numGroups,
                                                     uint64_t tmask, Offs32<R> off,
                                                     const cx<R>* __restrict__ mat) {
@@ -163,13 +180,13 @@ __global__ __launch_bounds__(BLOCK) void k_matk_reg(cx<R>* __restrict__ a, uint6
     const uint64_t base = ins0_mask(g, tmask);
     cx<R> v[D];
 #pragma unroll
-    for (int l = 0; l < D; l++) v[l] = a[base | off.o[l]];
+    for (int l = 0; l < D; l++) v[l] = ld<NT>(a + (base | off.o[l]));
 #pragma unroll
     for (int r = 0; r < D; r++) {
         cx<R> acc = mk<R>(0, 0);
 #pragma unroll
         for (int l = 0; l < D; l++) acc = cfma(acc, sm[r * D + l], v[l]);
-        a[base | off.o[r]] = acc;
+        st<NT>(a + (base | off.o[r]), acc);
     }
 }
 
@@ -309,15 +326,16 @@ template <typename R>
 void launch_mat1(hipStream_t s, cx<R>* a, unsigned nq, unsigned t, uint64_t ctrlMask, const Mat2<R>& g) {
     unsigned nc = (unsigned)__builtin_popcountll(ctrlMask);
     uint64_t numIts = (1ULL << nq) >> (nc + 1);
-    hipLaunchKernelGGL(k_mat1<R>, dim3(blocks_for(numIts, ITEMS)), dim3(BLOCK), 0, s, a, numIts, t,
-                       ctrlMask | (1ULL << t), ctrlMask, g);
+    const uint64_t qmask = ctrlMask | (1ULL << t);
+    auto* kern = (qmask & line_bits<R>()) ? k_mat1<R, false> : k_mat1<R, true>;
+    hipLaunchKernelGGL(kern, dim3(blocks_for(numIts, ITEMS)), dim3(BLOCK), 0, s, a, numIts, t, qmask, ctrlMask, g);
 }
 
 template <typename R>
 void launch_swap(hipStream_t s, cx<R>* a, unsigned nq, unsigned q1, unsigned q2) {
     uint64_t numIts = (1ULL << nq) >> 2;
-    hipLaunchKernelGGL(k_swap<R>, dim3(blocks_for(numIts, ITEMS)), dim3(BLOCK), 0, s, a, numIts,
-                       1ULL << q1, 1ULL << q2);
+    auto* kern = (((1ULL << q1) | (1ULL << q2)) & line_bits<R>()) ? k_swap<R, false> : k_swap<R, true>;
+    hipLaunchKernelGGL(kern, dim3(blocks_for(numIts, ITEMS)), dim3(BLOCK), 0, s, a, numIts, 1ULL << q1, 1ULL << q2);
 }
 
 template <typename R>
@@ -329,8 +347,8 @@ void launch_pauli(hipStream_t s, cx<R>* a, unsigned nq, uint64_t mxy, uint64_t m
     }
     unsigned hb = 63u - (unsigned)__builtin_clzll(mxy);
     uint64_t numIts = num >> 1;
-    hipLaunchKernelGGL(k_pauli<R>, dim3(blocks_for(numIts, ITEMS)), dim3(BLOCK), 0, s, a, numIts, hb,
-                       mxy, myz, th, fe);
+    auto* kern = (mxy & line_bits<R>()) ? k_pauli<R, false> : k_pauli<R, true>;
+    hipLaunchKernelGGL(kern, dim3(blocks_for(numIts, ITEMS)), dim3(BLOCK), 0, s, a, numIts, hb, mxy, myz, th, fe);
 }
 
 template <typename R>
@@ -358,13 +376,17 @@ void launch_matk(hipStream_t s, cx<R>* a, unsigned nq, const unsigned* targets, 
             off.o[l] = o;
         }
         dim3 grid((unsigned)((numGroups + BLOCK - 1) / BLOCK));
+        const bool nt = (tmask & line_bits<R>()) == 0;
+#define FSA_MATK(K) hipLaunchKernelGGL((nt ? k_matk_reg<R, K, true> : k_matk_reg<R, K, false>), grid, dim3(BLOCK), 0, s, \
+                                       a, numGroups, tmask, off, dmat)
         switch (k) {
-        case 1: hipLaunchKernelGGL((k_matk_reg<R, 1>), grid, dim3(BLOCK), 0, s, a, numGroups, tmask, off, dmat); break;
-        case 2: hipLaunchKernelGGL((k_matk_reg<R, 2>), grid, dim3(BLOCK), 0, s, a, numGroups, tmask, off, dmat); break;
-        case 3: hipLaunchKernelGGL((k_matk_reg<R, 3>), grid, dim3(BLOCK), 0, s, a, numGroups, tmask, off, dmat); break;
-        case 4: hipLaunchKernelGGL((k_matk_reg<R, 4>), grid, dim3(BLOCK), 0, s, a, numGroups, tmask, off, dmat); break;
-        default: hipLaunchKernelGGL((k_matk_reg<R, 5>), grid, dim3(BLOCK), 0, s, a, numGroups, tmask, off, dmat); break;
+        case 1: FSA_MATK(1); break;
+        case 2: FSA_MATK(2); break;
+        case 3: FSA_MATK(3); break;
+        case 4: FSA_MATK(4); break;
+        default: FSA_MATK(5); break;
         }
+#undef FSA_MATK
         return;
     }
     // offsets live after the transposed matrix in the same parameter block
