@@ -29,18 +29,30 @@ __host__ __device__ __forceinline__ cx<R> cadd(cx<R> a, cx<R> b) { return mk<R>(
 template <typename R>
 __host__ __device__ __forceinline__ cx<R> cscale(cx<R> a, R s) { return mk<R>(a.x * s, a.y * s); }
 
-// Non-temporal amplitude load / store (streamed state traffic that is not
-// reused within the kernel: 2-3% faster than cached accesses on gfx950)
+// Amplitude load / store for kernels that stream the state once: FP64 goes
+// non-temporal (one 16-byte access; 2-7% faster on gfx950 when each line is
+// written whole), FP32 stays cached (non-temporal measured slower there).
 template <typename R>
-__device__ __forceinline__ cx<R> ntl(const cx<R>* p) {
-    const R* q = reinterpret_cast<const R*>(p);
-    return mk<R>(__builtin_nontemporal_load(q), __builtin_nontemporal_load(q + 1));
+__device__ __forceinline__ cx<R> ld_stream(const cx<R>* p) {
+    if constexpr (sizeof(R) == 8) {
+        typedef R v2 __attribute__((ext_vector_type(2)));
+        const v2 v = __builtin_nontemporal_load(reinterpret_cast<const v2*>(p));
+        return mk<R>(v.x, v.y);
+    } else {
+        return *p;
+    }
 }
 template <typename R>
-__device__ __forceinline__ void nts(cx<R>* p, cx<R> v) {
-    R* q = reinterpret_cast<R*>(p);
-    __builtin_nontemporal_store(v.x, q);
-    __builtin_nontemporal_store(v.y, q + 1);
+__device__ __forceinline__ void st_stream(cx<R>* p, cx<R> v) {
+    if constexpr (sizeof(R) == 8) {
+        typedef R v2 __attribute__((ext_vector_type(2)));
+        v2 w;
+        w.x = v.x;
+        w.y = v.y;
+        __builtin_nontemporal_store(w, reinterpret_cast<v2*>(p));
+    } else {
+        *p = v;
+    }
 }
 // acc + a*b
 template <typename R>

@@ -139,7 +139,21 @@ __global__ __launch_bounds__(BLOCK) void k_pauli_element(const cx<R>* __restrict
 // over phi and lambda: the term <lambda| f P |phi> (as k_pauli_element), then
 // U^dag on both states with the arithmetic of k_pauli / k_pauli_diag / k_phase.
 // Each pair (j, j ^ mxy) of both states is read once and written once.
-template <typename R>
+// NT: streaming state access (ld_stream / st_stream) when every 128-byte line
+// a wave touches is written whole by one store instruction (no X/Y bit inside
+// a line)
+template <typename R, bool NT>
+__device__ __forceinline__ cx<R> adj_ld(const cx<R>* p) {
+    if constexpr (NT) return ld_stream(p);
+    else return *p;
+}
+template <typename R, bool NT>
+__device__ __forceinline__ void adj_st(cx<R>* p, cx<R> v) {
+    if constexpr (NT) st_stream(p, v);
+    else *p = v;
+}
+
+template <typename R, bool NT>
 __global__ __launch_bounds__(BLOCK) void k_adjoint_step(cx<R>* __restrict__ phi, cx<R>* __restrict__ lam,
                                                         uint64_t numIts, unsigned hb, AdjointStepDev st,
                                                         double* __restrict__ part) {
@@ -150,7 +164,8 @@ __global__ __launch_bounds__(BLOCK) void k_adjoint_step(cx<R>* __restrict__ phi,
     for (uint64_t k = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; k < numIts; k += (uint64_t)gridDim.x * BLOCK) {
         if (st.mxy) {
             const uint64_t j0 = ins0(k, hb), j1 = j0 ^ st.mxy;
-            const cx<R> p0 = phi[j0], p1 = phi[j1], l0 = lam[j0], l1 = lam[j1];
+            const cx<R> p0 = adj_ld<R, NT>(phi + j0), p1 = adj_ld<R, NT>(phi + j1);
+            const cx<R> l0 = adj_ld<R, NT>(lam + j0), l1 = adj_ld<R, NT>(lam + j1);
             const bool o0 = par64(j0 & st.myz) != 0, o1 = par64(j1 & st.myz) != 0;
             const double s0 = o0 ? -1.0 : 1.0, s1 = o1 ? -1.0 : 1.0;
             // conj(lambda_j) (-1)^{par(j' & myz)} phi_j' for j = j0 and j = j1
@@ -159,24 +174,24 @@ __global__ __launch_bounds__(BLOCK) void k_adjoint_step(cx<R>* __restrict__ phi,
             vi += s1 * ((double)l0.x * (double)p1.y - (double)l0.y * (double)p1.x) +
                   s0 * ((double)l1.x * (double)p0.y - (double)l1.y * (double)p0.x);
             const cx<R> f0 = o0 ? fo : fe, f1 = o1 ? fo : fe;
-            phi[j0] = cfma(cmul(th, p0), f1, p1);
-            phi[j1] = cfma(cmul(th, p1), f0, p0);
-            lam[j0] = cfma(cmul(th, l0), f1, l1);
-            lam[j1] = cfma(cmul(th, l1), f0, l0);
+            adj_st<R, NT>(phi + j0, cfma(cmul(th, p0), f1, p1));
+            adj_st<R, NT>(phi + j1, cfma(cmul(th, p1), f0, p0));
+            adj_st<R, NT>(lam + j0, cfma(cmul(th, l0), f1, l1));
+            adj_st<R, NT>(lam + j1, cfma(cmul(th, l1), f0, l0));
         } else {
-            const cx<R> p = phi[k], l = lam[k];
+            const cx<R> p = adj_ld<R, NT>(phi + k), l = adj_ld<R, NT>(lam + k);
             const bool odd = par64(k & st.myz) != 0;
             const double sg = odd ? -1.0 : 1.0;
             vr += sg * ((double)l.x * (double)p.x + (double)l.y * (double)p.y);
             vi += sg * ((double)l.x * (double)p.y - (double)l.y * (double)p.x);
             if (st.phase) {
                 const cx<R> f = odd ? e1 : e0;
-                phi[k] = cmul(p, f);
-                lam[k] = cmul(l, f);
+                adj_st<R, NT>(phi + k, cmul(p, f));
+                adj_st<R, NT>(lam + k, cmul(l, f));
             } else {
                 const cx<R> f = odd ? fo : fe;
-                phi[k] = cfma(cmul(th, p), f, p);
-                lam[k] = cfma(cmul(th, l), f, l);
+                adj_st<R, NT>(phi + k, cfma(cmul(th, p), f, p));
+                adj_st<R, NT>(lam + k, cfma(cmul(th, l), f, l));
             }
         }
     }
@@ -320,7 +335,9 @@ void launch_adjoint_step(hipStream_t s, cx<R>* phi, cx<R>* lam, unsigned nq, con
     const uint64_t numIts = st.mxy ? 1ULL << (nq - 1) : 1ULL << nq;
     const unsigned hb = st.mxy ? 63u - (unsigned)__builtin_clzll(st.mxy) : 0u;
     unsigned nb = red_blocks(numIts);
-    hipLaunchKernelGGL(k_adjoint_step<R>, dim3(nb), dim3(BLOCK), 0, s, phi, lam, numIts, hb, st, ws);
+    const uint64_t lineBits = 128 / sizeof(cx<R>) - 1;
+    auto* kern = (st.mxy & lineBits) ? k_adjoint_step<R, false> : k_adjoint_step<R, true>;
+    hipLaunchKernelGGL(kern, dim3(nb), dim3(BLOCK), 0, s, phi, lam, numIts, hb, st, ws);
     hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(BLOCK), 0, s, ws, nb, dout);
 }
 

@@ -14,17 +14,18 @@ namespace {
 
 constexpr int ITEMS = 4;
 
-// Amplitude access, non-temporal (NT) when every 128-byte line a wave touches
-// is written whole by one store instruction; kernels whose mixed bits fall
-// inside a line write alternating elements per instruction and stay cached.
+// Amplitude access: the streaming form (ld_stream / st_stream) when every
+// 128-byte line a wave touches is written whole by one store instruction
+// (NT); kernels whose mixed bits fall inside a line write alternating
+// elements per instruction and stay on plain cached accesses.
 template <bool NT, typename R>
 __device__ __forceinline__ cx<R> ld(const cx<R>* p) {
-    if constexpr (NT) return ntl(p);
+    if constexpr (NT) return ld_stream(p);
     else return *p;
 }
 template <bool NT, typename R>
 __device__ __forceinline__ void st(cx<R>* p, cx<R> v) {
-    if constexpr (NT) nts(p, v);
+    if constexpr (NT) st_stream(p, v);
     else *p = v;
 }
 // amplitude-index bits inside one 128-byte line
@@ -125,14 +126,14 @@ __global__ __launch_bounds__(BLOCK) void k_pauli_diag(cx<R>* __restrict__ a, uin
 #pragma unroll
     for (int p = 0; p < ITEMS; p++) {
         uint64_t j = first + (uint64_t)p * BLOCK;
-        if (j < num) v[p] = ntl(a + j);
+        if (j < num) v[p] = ld_stream(a + j);
     }
 #pragma unroll
     for (int p = 0; p < ITEMS; p++) {
         uint64_t j = first + (uint64_t)p * BLOCK;
         if (j < num) {
             cx<R> f = par64(j & myz) ? fo : fe;
-            nts(a + j, cfma(cmul(th, v[p]), f, v[p]));
+            st_stream(a + j, cfma(cmul(th, v[p]), f, v[p]));
         }
     }
 }
@@ -146,12 +147,12 @@ __global__ __launch_bounds__(BLOCK) void k_phase(cx<R>* __restrict__ a, uint64_t
 #pragma unroll
     for (int p = 0; p < ITEMS; p++) {
         uint64_t j = first + (uint64_t)p * BLOCK;
-        if (j < num) v[p] = ntl(a + j);
+        if (j < num) v[p] = ld_stream(a + j);
     }
 #pragma unroll
     for (int p = 0; p < ITEMS; p++) {
         uint64_t j = first + (uint64_t)p * BLOCK;
-        if (j < num) nts(a + j, cmul(v[p], par64(j & mask) ? f1 : f0));
+        if (j < num) st_stream(a + j, cmul(v[p], par64(j & mask) ? f1 : f0));
     }
 }
 
@@ -161,7 +162,7 @@ __global__ __launch_bounds__(BLOCK) void k_scale(cx<R>* __restrict__ a, uint64_t
 #pragma unroll
     for (int p = 0; p < ITEMS; p++) {
         uint64_t j = first + (uint64_t)p * BLOCK;
-        if (j < num) nts(a + j, cscale(ntl(a + j), f));
+        if (j < num) st_stream(a + j, cscale(ld_stream(a + j), f));
     }
 }
 
