@@ -145,7 +145,8 @@ struct BlobBuilder {
             bs.m = (uint32_t)s.tq.size();
             bs.item0 = (uint32_t)items.size();
             uint32_t ngrad = 0;
-            for (int oi : s.ops) {
+            for (size_t ii = 0; ii < s.ops.size(); ii++) {
+                const int oi = s.ops[ii];
                 const HOp& op = ops[oi];
                 TileOp t = translate_item<R>(op, pos, bs.tileMask, mats);
                 if (op.kind == OP_GRAD && ngrad == MAX_STAGE_GRADS_HOST) {
@@ -159,6 +160,13 @@ struct BlobBuilder {
                     t.p[0] = ngrad++;
                     t.mat_off = numJobs + (uint32_t)op.param;
                     t.param = NO_PARAM;
+                    // the gadget's inverse right after it: one fused sweep (item field k = 1)
+                    if (ii + 1 < s.ops.size()) {
+                        const int ni = s.ops[ii + 1];
+                        const HOp& nx = ops[ni];
+                        if ((nx.kind == OP_PAULI || nx.kind == OP_PHASE) && nx.param == op.param && sign[ni] < 0)
+                            t.k = 1;
+                    }
                 } else if (op.param >= 0 && (op.kind == OP_PAULI || op.kind == OP_PHASE)) {
                     t.param = (uint32_t)op.param;
                     t.f[4] = op.pw.real();

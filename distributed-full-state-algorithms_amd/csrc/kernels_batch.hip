@@ -92,6 +92,63 @@ __device__ __forceinline__ void grad_partial(const cx<R>* phi, const cx<R>* lam,
     vi = op.f[0] * ai + op.f[1] * ar;
 }
 
+// A gradient item fused with the inverse gadget after it (same Pauli masks):
+// one sweep over the pairs (l, l ^ mxy) of both tiles accumulates the item's
+// <lambda|P|phi> terms (as grad_partial) and applies the gadget `o` to phi and
+// lambda with the arithmetic of tile_apply's OP_PAULI / OP_PHASE.
+template <typename R, int TB>
+__device__ __forceinline__ void grad_apply(cx<R>* phi, cx<R>* lam, uint32_t T, uint64_t base, const TileOp& g,
+                                           const TileOp& o, double& vr, double& vi) {
+    const uint32_t bp = par64(base & g.gmask);
+    const uint32_t lxy = g.lmask, lyz = g.lmask2;
+    double ar = 0, ai = 0;
+    auto term = [&](const cx<R>& x, const cx<R>& y, uint32_t k) {   // conj(x) y (-1)^{par(k & myz)}
+        const double sg = (bp ^ par32(k & lyz)) ? -1.0 : 1.0;
+        ar += sg * ((double)x.x * (double)y.x + (double)x.y * (double)y.y);
+        ai += sg * ((double)x.x * (double)y.y - (double)x.y * (double)y.x);
+    };
+    if (o.kind == OP_PHASE) {
+        const cx<R> f0 = ldc<R>(o.f, 0), f1 = ldc<R>(o.f, 1);
+        const uint32_t bq = par64(base & o.gmask);
+        for (uint32_t l = threadIdx.x; l < T; l += TB) {
+            const cx<R> x = lam[l], y = phi[l];
+            term(x, y, l);
+            const cx<R> f = (bq ^ par32(l & o.lmask)) ? f1 : f0;
+            phi[l] = cmul(y, f);
+            lam[l] = cmul(x, f);
+        }
+    } else {
+        const cx<R> th = ldc<R>(o.f, 0), fe = ldc<R>(o.f, 1);
+        const cx<R> fo = mk<R>(-fe.x, -fe.y);
+        const uint32_t bq = par64(base & o.gmask);
+        if (lxy == 0) {
+            for (uint32_t l = threadIdx.x; l < T; l += TB) {
+                const cx<R> x = lam[l], y = phi[l];
+                term(x, y, l);
+                const cx<R> f = (bq ^ par32(l & o.lmask2)) ? fo : fe;
+                phi[l] = cfma(cmul(th, y), f, y);
+                lam[l] = cfma(cmul(th, x), f, x);
+            }
+        } else {
+            const unsigned hb = 31u - (unsigned)__builtin_clz(lxy);
+            for (uint32_t j = threadIdx.x; j < (T >> 1); j += TB) {
+                const uint32_t i0 = ins0_32(j, hb), i1 = i0 ^ lxy;
+                const cx<R> p0 = phi[i0], p1 = phi[i1], l0 = lam[i0], l1 = lam[i1];
+                term(l0, p1, i1);
+                term(l1, p0, i0);
+                const cx<R> f0 = (bq ^ par32(i0 & o.lmask2)) ? fo : fe;
+                const cx<R> f1 = (bq ^ par32(i1 & o.lmask2)) ? fo : fe;
+                phi[i0] = cfma(cmul(th, p0), f1, p1);
+                phi[i1] = cfma(cmul(th, p1), f0, p0);
+                lam[i0] = cfma(cmul(th, l0), f1, l1);
+                lam[i1] = cfma(cmul(th, l1), f0, l0);
+            }
+        }
+    }
+    vr = g.f[0] * ar - g.f[1] * ai;
+    vi = g.f[0] * ai + g.f[1] * ar;
+}
+
 // lambda_j = sum_t f_t (-1)^{par(k & myz_t)} phi_k, k = j ^ mxy_t, for j in [j0, j0 + T);
 // accumulates <phi|lambda> over those j
 template <typename R, int TB>
@@ -194,14 +251,23 @@ __device__ __forceinline__ void run_items(cx<R>* phi, cx<R>* lam, uint32_t T, ui
         }
         if (REV && op.kind == OP_GRAD) {
             double vr, vi;
-            grad_partial<R, TB>(phi, lam, T, base, op, vr, vi);
+            const bool fused = op.k == 1u;  // the next item is this gadget's inverse
+            if (fused) {
+                TileOp inv = items[i + 1];
+                bind_param(inv, params);
+                grad_apply<R, TB>(phi, lam, T, base, op, inv, vr, vi);
+                i++;
+            } else {
+                grad_partial<R, TB>(phi, lam, T, base, op, vr, vi);
+            }
             bsum2<TB>(vr, vi, red);
             if (threadIdx.x == 0) {
                 double* r = gacc + 2 * op.p[0];
                 if (firstTile) { r[0] = vr; r[1] = vi; }
                 else { r[0] += vr; r[1] += vi; }
             }
-            continue;                       // no barrier needed: nothing was written to the tiles
+            if (fused) __syncthreads();     // the tiles were written (a gradient sum alone only reads them)
+            continue;
         }
         bind_param(op, params);
         tile_apply<R, TB, 1>(phi, T, base, op, mats);
