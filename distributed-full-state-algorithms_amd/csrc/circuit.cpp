@@ -231,12 +231,48 @@ void adjoint_impl(Circuit& c, State& psi0, const double* coeffs, unsigned T, con
     // and on lambda in one pass over both states (k_adjoint_step); any other op
     // is undone on each state by its own kernel.  (The tile-pass form of this
     // sweep, build_reverse_plan + launch_rev_pass, is not used by this entry point.)
-    size_t red = reduce_workspace_doubles(phi.s.num());
+    // Runs of consecutive parametrised diagonal gadgets (phase gadgets, Z-only
+    // Pauli gadgets) take one pass per ADJ_DIAG_MAX of them (k_adjoint_diag).
+    size_t red = reduce_workspace_doubles(phi.s.num()) * ADJ_DIAG_MAX;
     double* ws = phi.s.workspace(red + 2 * (P + 1) + 8);
     double* res = ws + red;
     launch_inner<R>(st, a_phi, a_lam, phi.s.num(), ws, res);
+    auto diag_gadget = [](const HOp& o) {
+        return o.param >= 0 && (o.kind == OP_PHASE || (o.kind == OP_PAULI && o.mxy == 0));
+    };
     for (size_t k = c.ops.size(); k-- > 0;) {
         const HOp& op = c.ops[k];
+        if (diag_gadget(op) && k > 0 && diag_gadget(c.ops[k - 1])) {
+            AdjointDiagDev d;
+            std::memset(&d, 0, sizeof(d));
+            double* outs[ADJ_DIAG_MAX];
+            size_t j = k + 1;
+            while (j-- > 0 && d.count < ADJ_DIAG_MAX && diag_gadget(c.ops[j])) {
+                const HOp& g = c.ops[j];
+                const HOp gi = inverse(g);
+                const unsigned i = d.count++;
+                cd e0, e1;
+                if (g.kind == OP_PAULI) {              // th v + (+-fe) v, f = i^{#Y} (1: no Y)
+                    d.myz[i] = g.myz;
+                    d.fr[i] = g.pw.real(); d.fi[i] = g.pw.imag();
+                    e0 = gi.th + gi.fe;
+                    e1 = gi.th - gi.fe;
+                } else {
+                    d.myz[i] = g.pmask;
+                    d.fr[i] = 1.0; d.fi[i] = 0.0;
+                    e0 = cd(std::cos(gi.theta), std::sin(gi.theta));
+                    e1 = cd(std::cos(gi.theta), -std::sin(gi.theta));
+                }
+                d.e0r[i] = e0.real(); d.e0i[i] = e0.imag();
+                d.e1r[i] = e1.real(); d.e1i[i] = e1.imag();
+                outs[i] = res + 2 * (1 + g.param);
+            }
+            ProfScope prof("adjoint_diag", st);
+            launch_adjoint_diag<R>(st, a_phi, a_lam, n, d, ws, outs);
+            hip_check(hipGetLastError(), "adjoint diagonal run launch");
+            k -= d.count - 1;       // the loop's k-- moves past the run
+            continue;
+        }
         const HOp inv = inverse(op);
         if (op.param >= 0) {
             AdjointStepDev d;

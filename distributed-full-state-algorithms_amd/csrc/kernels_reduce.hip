@@ -202,6 +202,51 @@ __global__ __launch_bounds__(BLOCK) void k_adjoint_step(cx<R>* __restrict__ phi,
     }
 }
 
+// A run of diagonal gadgets (AdjointDiagDev): conj(lambda_k) phi_k is formed
+// once per amplitude and added, with gadget g's parity sign, to g's FP64
+// accumulators; both states are multiplied by the product of the gadgets'
+// U^dag factors.  Each accumulator then takes the same two-level reduction as
+// k_adjoint_step: block sums here (rows g * gridDim.x + block of part), then
+// k_final_sum over the rows of one gadget.
+template <typename R>
+__global__ __launch_bounds__(BLOCK) void k_adjoint_diag(cx<R>* __restrict__ phi, cx<R>* __restrict__ lam,
+                                                        uint64_t num, AdjointDiagDev st, double* __restrict__ part) {
+    double vr[ADJ_DIAG_MAX], vi[ADJ_DIAG_MAX];
+#pragma unroll
+    for (unsigned g = 0; g < ADJ_DIAG_MAX; g++) { vr[g] = 0; vi[g] = 0; }
+    const unsigned cnt = st.count;
+    for (uint64_t k = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; k < num; k += (uint64_t)gridDim.x * BLOCK) {
+        const cx<R> p = adj_ld<R, true>(phi + k), l = adj_ld<R, true>(lam + k);
+        const double wr = (double)l.x * (double)p.x + (double)l.y * (double)p.y;
+        const double wi = (double)l.x * (double)p.y - (double)l.y * (double)p.x;
+        cx<R> f = mk<R>((R)1, (R)0);
+#pragma unroll
+        for (unsigned g = 0; g < ADJ_DIAG_MAX; g++) {
+            if (g < cnt) {
+                const bool odd = par64(k & st.myz[g]) != 0;
+                vr[g] += odd ? -wr : wr;
+                vi[g] += odd ? -wi : wi;
+                f = cmul(f, odd ? mk<R>((R)st.e1r[g], (R)st.e1i[g]) : mk<R>((R)st.e0r[g], (R)st.e0i[g]));
+            }
+        }
+        adj_st<R, true>(phi + k, cmul(p, f));
+        adj_st<R, true>(lam + k, cmul(l, f));
+    }
+#pragma unroll
+    for (unsigned g = 0; g < ADJ_DIAG_MAX; g++) {
+        if (g < cnt) {
+            double a = vr[g], b = vi[g];
+            block_sum2(a, b);
+            if (threadIdx.x == 0) {
+                double* row = part + 2 * ((size_t)g * gridDim.x + blockIdx.x);
+                row[0] = st.fr[g] * a - st.fi[g] * b;
+                row[1] = st.fr[g] * b + st.fi[g] * a;
+            }
+            __syncthreads();    // block_sum2's wave partials are reused by the next gadget
+        }
+    }
+}
+
 // out[j] = sum_t f_t (-1)^{par(j1 & myz_t)} in[j1], j1 = j ^ mxy_t
 template <typename R>
 __global__ __launch_bounds__(BLOCK) void k_apply_pauli_sum(const cx<R>* __restrict__ in, cx<R>* __restrict__ out,
@@ -341,6 +386,16 @@ void launch_adjoint_step(hipStream_t s, cx<R>* phi, cx<R>* lam, unsigned nq, con
     hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(BLOCK), 0, s, ws, nb, dout);
 }
 
+template <typename R>
+void launch_adjoint_diag(hipStream_t s, cx<R>* phi, cx<R>* lam, unsigned nq, const AdjointDiagDev& st, double* ws,
+                         double* const* dout) {
+    const uint64_t num = 1ULL << nq;
+    const unsigned nb = red_blocks(num);
+    hipLaunchKernelGGL(k_adjoint_diag<R>, dim3(nb), dim3(BLOCK), 0, s, phi, lam, num, st, ws);
+    for (unsigned g = 0; g < st.count; g++)
+        hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(BLOCK), 0, s, ws + 2 * (size_t)g * nb, nb, dout[g]);
+}
+
 // ws must hold 2 * numTerms * ceil(2^nbits / EXP_CHUNK) doubles
 template <typename R>
 void launch_dm_expec(hipStream_t s, const cx<R>* rho, unsigned N, const PauliTermDev* dterms,
@@ -421,6 +476,8 @@ void launch_f32_to_f64(hipStream_t s, const cx<float>* in, cx<double>* out, uint
     template void launch_inner<R>(hipStream_t, const cx<R>*, const cx<R>*, uint64_t, double*, double*);       \
     template void launch_adjoint_step<R>(hipStream_t, cx<R>*, cx<R>*, unsigned, const AdjointStepDev&, double*,  \
                                          double*);                                                              \
+    template void launch_adjoint_diag<R>(hipStream_t, cx<R>*, cx<R>*, unsigned, const AdjointDiagDev&, double*,  \
+                                         double* const*);                                                       \
     template void launch_pauli_element<R>(hipStream_t, const cx<R>*, const cx<R>*, uint64_t, PauliTermDev,     \
                                           double*, double*);                                                  \
     template void launch_dm_expec<R>(hipStream_t, const cx<R>*, unsigned, const PauliTermDev*, unsigned,       \

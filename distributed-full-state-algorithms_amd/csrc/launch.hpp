@@ -86,6 +86,23 @@ struct AdjointStepDev {
 template <typename R>
 void launch_adjoint_step(hipStream_t s, cx<R>* phi, cx<R>* lam, unsigned nq, const AdjointStepDev& st, double* ws,
                          double* dout);
+// a run of consecutive diagonal gadgets of the reverse sweep in one pass
+// (kernels_reduce.hip): every term <lambda|f_g P_g|phi> is taken from the
+// run's input states (diagonal unitaries leave <D lambda|P|D phi> unchanged
+// for diagonal P), then every gadget's U^dag factor is applied to both states
+constexpr unsigned ADJ_DIAG_MAX = 8;
+struct AdjointDiagDev {
+    uint64_t myz[ADJ_DIAG_MAX];                       // parity masks
+    double fr[ADJ_DIAG_MAX], fi[ADJ_DIAG_MAX];        // term factors f_g
+    double e0r[ADJ_DIAG_MAX], e0i[ADJ_DIAG_MAX];      // U^dag factor on even parity
+    double e1r[ADJ_DIAG_MAX], e1i[ADJ_DIAG_MAX];      // and on odd parity
+    uint32_t count;                                   // gadgets in the run (1..ADJ_DIAG_MAX)
+    uint32_t pad;
+};
+// ws: ADJ_DIAG_MAX * reduce_workspace_doubles doubles; dout[g]: 2 doubles each
+template <typename R>
+void launch_adjoint_diag(hipStream_t s, cx<R>* phi, cx<R>* lam, unsigned nq, const AdjointDiagDev& st, double* ws,
+                         double* const* dout);
 template <typename R>
 void launch_pauli_element(hipStream_t s, const cx<R>* a, const cx<R>* b, uint64_t num, PauliTermDev term,
                           double* ws, double* dout);
