@@ -233,19 +233,54 @@ void adjoint_impl(Circuit& c, State& psi0, const double* coeffs, unsigned T, con
     // sweep, build_reverse_plan + launch_rev_pass, is not used by this entry point.)
     // Runs of consecutive parametrised diagonal gadgets (phase gadgets, Z-only
     // Pauli gadgets) take one pass per ADJ_DIAG_MAX of them (k_adjoint_diag).
-    size_t red = reduce_workspace_doubles(phi.s.num()) * ADJ_DIAG_MAX;
-    double* ws = phi.s.workspace(red + 2 * (P + 1) + 8);
+    // Every gadget's partial sums go to the row slot of its parameter; one
+    // launch adds each slot's rows after the sweep (launch_grad_sums).
+    const size_t red = reduce_workspace_doubles(phi.s.num());
+    const uint64_t stride = adjoint_row_stride(n);
+    double* ws = phi.s.workspace(red + 2 * (P + 1) + stride * P + 8);
     double* res = ws + red;
+    double* rows = res + 2 * (P + 1);
+    std::vector<uint32_t> nbs(P, 0);
     launch_inner<R>(st, a_phi, a_lam, phi.s.num(), ws, res);
     auto diag_gadget = [](const HOp& o) {
         return o.param >= 0 && (o.kind == OP_PHASE || (o.kind == OP_PAULI && o.mxy == 0));
     };
+    // Runs of consecutive parametrised one-qubit X/Y gadgets on distinct
+    // qubits take one pass per ADJ_QRUN_MAX of them (k_adjoint_qrun).
+    auto qubit_gadget = [](const HOp& o) {
+        return o.param >= 0 && o.kind == OP_PAULI && __builtin_popcountll(o.mxy) == 1 && (o.myz & ~o.mxy) == 0;
+    };
     for (size_t k = c.ops.size(); k-- > 0;) {
         const HOp& op = c.ops[k];
+        if (qubit_gadget(op) && k > 0 && qubit_gadget(c.ops[k - 1]) && c.ops[k - 1].mxy != op.mxy) {
+            AdjointQubitRunDev d;
+            std::memset(&d, 0, sizeof(d));
+            uint64_t used = 0;
+            size_t j = k + 1;
+            while (j-- > 0 && d.count < ADJ_QRUN_MAX && qubit_gadget(c.ops[j]) && !(c.ops[j].mxy & used)) {
+                const HOp& g = c.ops[j];
+                const HOp gi = inverse(g);
+                const unsigned i = d.count++;
+                used |= g.mxy;
+                d.q[i] = (uint32_t)__builtin_ctzll(g.mxy);
+                d.y[i] = g.myz != 0;
+                d.fr[i] = g.pw.real(); d.fi[i] = g.pw.imag();
+                d.thr[i] = gi.th.real(); d.thi[i] = gi.th.imag();
+                d.fer[i] = gi.fe.real(); d.fei[i] = gi.fe.imag();
+                d.slot[i] = (uint32_t)g.param;
+            }
+            std::copy(d.q, d.q + d.count, d.sorted);
+            std::sort(d.sorted, d.sorted + d.count);
+            ProfScope prof("adjoint_qrun", st);
+            const unsigned nb = launch_adjoint_qrun<R>(st, a_phi, a_lam, n, d, rows, stride);
+            hip_check(hipGetLastError(), "adjoint qubit run launch");
+            for (unsigned i = 0; i < d.count; i++) nbs[d.slot[i]] = nb;
+            k -= d.count - 1;       // the loop's k-- moves past the run
+            continue;
+        }
         if (diag_gadget(op) && k > 0 && diag_gadget(c.ops[k - 1])) {
             AdjointDiagDev d;
             std::memset(&d, 0, sizeof(d));
-            double* outs[ADJ_DIAG_MAX];
             size_t j = k + 1;
             while (j-- > 0 && d.count < ADJ_DIAG_MAX && diag_gadget(c.ops[j])) {
                 const HOp& g = c.ops[j];
@@ -265,11 +300,12 @@ void adjoint_impl(Circuit& c, State& psi0, const double* coeffs, unsigned T, con
                 }
                 d.e0r[i] = e0.real(); d.e0i[i] = e0.imag();
                 d.e1r[i] = e1.real(); d.e1i[i] = e1.imag();
-                outs[i] = res + 2 * (1 + g.param);
+                d.slot[i] = (uint32_t)g.param;
             }
             ProfScope prof("adjoint_diag", st);
-            launch_adjoint_diag<R>(st, a_phi, a_lam, n, d, ws, outs);
+            const unsigned nb = launch_adjoint_diag<R>(st, a_phi, a_lam, n, d, rows, stride);
             hip_check(hipGetLastError(), "adjoint diagonal run launch");
+            for (unsigned i = 0; i < d.count; i++) nbs[d.slot[i]] = nb;
             k -= d.count - 1;       // the loop's k-- moves past the run
             continue;
         }
@@ -289,14 +325,20 @@ void adjoint_impl(Circuit& c, State& psi0, const double* coeffs, unsigned T, con
                 d.e0r = std::cos(inv.theta); d.e0i = std::sin(inv.theta);
                 d.e1r = std::cos(inv.theta); d.e1i = -std::sin(inv.theta);
             }
+            d.slot = (uint32_t)op.param;
             ProfScope prof("adjoint_step", st);
-            launch_adjoint_step<R>(st, a_phi, a_lam, n, d, ws, res + 2 * (1 + op.param));
+            nbs[op.param] = launch_adjoint_step<R>(st, a_phi, a_lam, n, d, rows, stride);
             hip_check(hipGetLastError(), "adjoint step launch");
             continue;
         }
         std::vector<HOp> invs{inv};
         execute_now(phi.s, invs);
         execute_now(lam.s, invs);
+    }
+    if (P) {
+        auto* dnb = (const uint32_t*)phi.s.ring.push(nbs.data(), nbs.size() * sizeof(uint32_t));
+        launch_grad_sums(st, rows, stride, dnb, P, res + 2);
+        hip_check(hipGetLastError(), "gradient sums launch");
     }
     std::vector<double> host(2 * (P + 1));
     hip_check(hipMemcpyAsync(host.data(), res, host.size() * sizeof(double), hipMemcpyDeviceToHost, st), "hipMemcpyAsync");

@@ -4,6 +4,7 @@
 // Reductions are two-level and deterministic: a fixed grid writes one partial
 // per workgroup (wavefront shuffle, then LDS across the 4 waves), and a single
 // workgroup sums the partials in index order.  Results never depend on timing.
+#include "engine.hpp"
 #include "launch.hpp"
 
 namespace fsa {
@@ -48,6 +49,18 @@ __global__ __launch_bounds__(BLOCK) void k_final_sum(const double* __restrict__ 
     for (unsigned i = threadIdx.x; i < numParts; i += BLOCK) { vr += part[2 * i]; vi += part[2 * i + 1]; }
     block_sum2(vr, vi);
     if (threadIdx.x == 0) { out[0] = vr; out[1] = vi; }
+}
+
+// per-slot sums of the adjoint sweep's partial rows: block p adds the first
+// nb[p] rows of slot p in row order (the order k_final_sum uses)
+__global__ __launch_bounds__(BLOCK) void k_grad_sums(const double* __restrict__ rows, uint64_t stride,
+                                                     const uint32_t* __restrict__ nb, double* __restrict__ res) {
+    const double* r = rows + stride * blockIdx.x;
+    const unsigned n = nb[blockIdx.x];
+    double vr = 0, vi = 0;
+    for (unsigned i = threadIdx.x; i < n; i += BLOCK) { vr += r[2 * i]; vi += r[2 * i + 1]; }
+    block_sum2(vr, vi);
+    if (threadIdx.x == 0) { res[2 * blockIdx.x] = vr; res[2 * blockIdx.x + 1] = vi; }
 }
 
 template <typename R>
@@ -156,7 +169,7 @@ __device__ __forceinline__ void adj_st(cx<R>* p, cx<R> v) {
 template <typename R, bool NT>
 __global__ __launch_bounds__(BLOCK) void k_adjoint_step(cx<R>* __restrict__ phi, cx<R>* __restrict__ lam,
                                                         uint64_t numIts, unsigned hb, AdjointStepDev st,
-                                                        double* __restrict__ part) {
+                                                        double* __restrict__ rows, uint64_t stride) {
     const cx<R> th = mk<R>((R)st.thr, (R)st.thi), fe = mk<R>((R)st.fer, (R)st.fei);
     const cx<R> fo = mk<R>(-fe.x, -fe.y);
     const cx<R> e0 = mk<R>((R)st.e0r, (R)st.e0i), e1 = mk<R>((R)st.e1r, (R)st.e1i);
@@ -197,38 +210,51 @@ __global__ __launch_bounds__(BLOCK) void k_adjoint_step(cx<R>* __restrict__ phi,
     }
     block_sum2(vr, vi);
     if (threadIdx.x == 0) {
-        part[2 * blockIdx.x] = st.fr * vr - st.fi * vi;
-        part[2 * blockIdx.x + 1] = st.fr * vi + st.fi * vr;
+        double* row = rows + stride * st.slot + 2 * blockIdx.x;
+        row[0] = st.fr * vr - st.fi * vi;
+        row[1] = st.fr * vi + st.fi * vr;
     }
 }
 
 // A run of diagonal gadgets (AdjointDiagDev): conj(lambda_k) phi_k is formed
 // once per amplitude and added, with gadget g's parity sign, to g's FP64
 // accumulators; both states are multiplied by the product of the gadgets'
-// U^dag factors.  Each accumulator then takes the same two-level reduction as
-// k_adjoint_step: block sums here (rows g * gridDim.x + block of part), then
-// k_final_sum over the rows of one gadget.
-template <typename R>
+// U^dag factors, looked up by the amplitude's parity pattern (bit g = parity
+// of gadget g) in a table each block builds in LDS.  Each accumulator then
+// takes the same two-level reduction as k_adjoint_step: block sums here (one
+// row per workgroup in the gadget's slot), then k_grad_sums.
+template <typename R, bool WIDE>
 __global__ __launch_bounds__(BLOCK) void k_adjoint_diag(cx<R>* __restrict__ phi, cx<R>* __restrict__ lam,
-                                                        uint64_t num, AdjointDiagDev st, double* __restrict__ part) {
+                                                        uint64_t num, AdjointDiagDev st, double* __restrict__ rows,
+                                                        uint64_t stride) {
+    __shared__ cx<R> tab[1u << ADJ_DIAG_MAX];
+    const unsigned cnt = st.count;
+    for (unsigned t = threadIdx.x; t < (1u << cnt); t += BLOCK) {
+        cx<R> f = mk<R>((R)1, (R)0);
+        for (unsigned g = 0; g < cnt; g++)
+            f = cmul(f, ((t >> g) & 1u) ? mk<R>((R)st.e1r[g], (R)st.e1i[g]) : mk<R>((R)st.e0r[g], (R)st.e0i[g]));
+        tab[t] = f;
+    }
+    __syncthreads();
     double vr[ADJ_DIAG_MAX], vi[ADJ_DIAG_MAX];
 #pragma unroll
     for (unsigned g = 0; g < ADJ_DIAG_MAX; g++) { vr[g] = 0; vi[g] = 0; }
-    const unsigned cnt = st.count;
     for (uint64_t k = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; k < num; k += (uint64_t)gridDim.x * BLOCK) {
         const cx<R> p = adj_ld<R, true>(phi + k), l = adj_ld<R, true>(lam + k);
         const double wr = (double)l.x * (double)p.x + (double)l.y * (double)p.y;
         const double wi = (double)l.x * (double)p.y - (double)l.y * (double)p.x;
-        cx<R> f = mk<R>((R)1, (R)0);
+        unsigned pat = 0;
 #pragma unroll
         for (unsigned g = 0; g < ADJ_DIAG_MAX; g++) {
             if (g < cnt) {
-                const bool odd = par64(k & st.myz[g]) != 0;
+                // WIDE: states of more than 32 qubits (64-bit parities)
+                const unsigned odd = WIDE ? par64(k & st.myz[g]) : par32((uint32_t)k & (uint32_t)st.myz[g]);
+                pat |= odd << g;
                 vr[g] += odd ? -wr : wr;
                 vi[g] += odd ? -wi : wi;
-                f = cmul(f, odd ? mk<R>((R)st.e1r[g], (R)st.e1i[g]) : mk<R>((R)st.e0r[g], (R)st.e0i[g]));
             }
         }
+        const cx<R> f = tab[pat];
         adj_st<R, true>(phi + k, cmul(p, f));
         adj_st<R, true>(lam + k, cmul(l, f));
     }
@@ -238,12 +264,85 @@ __global__ __launch_bounds__(BLOCK) void k_adjoint_diag(cx<R>* __restrict__ phi,
             double a = vr[g], b = vi[g];
             block_sum2(a, b);
             if (threadIdx.x == 0) {
-                double* row = part + 2 * ((size_t)g * gridDim.x + blockIdx.x);
+                double* row = rows + stride * st.slot[g] + 2 * blockIdx.x;
                 row[0] = st.fr[g] * a - st.fi[g] * b;
                 row[1] = st.fr[g] * b + st.fi[g] * a;
             }
             __syncthreads();    // block_sum2's wave partials are reused by the next gadget
         }
+    }
+}
+
+// A run of G one-qubit X/Y gadgets on distinct qubits (AdjointQubitRunDev):
+// each thread takes groups of 2^G amplitudes of phi and lambda (bit g of the
+// group index = gadget g's qubit), adds every gadget's term from the loaded
+// values (conj(lambda_j) s(j') phi_j' over its pairs j' = j ^ bit, as in
+// k_adjoint_step), then applies every gadget's U^dag to its pairs and writes
+// the group back.  Reduction as in k_adjoint_diag.
+template <typename R, int G, bool NT>
+__global__ __launch_bounds__(BLOCK) void k_adjoint_qrun(cx<R>* __restrict__ phi, cx<R>* __restrict__ lam,
+                                                        uint64_t numGroups, AdjointQubitRunDev st,
+                                                        double* __restrict__ rows, uint64_t stride) {
+    constexpr unsigned E = 1u << G;
+    double vr[G], vi[G];
+#pragma unroll
+    for (int g = 0; g < G; g++) { vr[g] = 0; vi[g] = 0; }
+    uint64_t off[E];
+#pragma unroll
+    for (unsigned e = 0; e < E; e++) {
+        off[e] = 0;
+#pragma unroll
+        for (int g = 0; g < G; g++) if ((e >> g) & 1u) off[e] |= 1ULL << st.q[g];
+    }
+    for (uint64_t t = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; t < numGroups; t += (uint64_t)gridDim.x * BLOCK) {
+        uint64_t x = t;
+#pragma unroll
+        for (int i = 0; i < G; i++) x = ins0(x, st.sorted[i]);
+        cx<R> p[E], l[E];
+#pragma unroll
+        for (unsigned e = 0; e < E; e++) { p[e] = adj_ld<R, NT>(phi + (x | off[e])); l[e] = adj_ld<R, NT>(lam + (x | off[e])); }
+#pragma unroll
+        for (int g = 0; g < G; g++) {
+            // s(j) = -1 where the gadget is Y and j has its qubit set (the pair's second element)
+            const double s1 = st.y[g] ? -1.0 : 1.0;
+#pragma unroll
+            for (unsigned e = 0; e < E; e++) {
+                if ((e >> g) & 1u) continue;
+                const unsigned e1 = e | (1u << g);
+                vr[g] += s1 * ((double)l[e].x * (double)p[e1].x + (double)l[e].y * (double)p[e1].y) +
+                         ((double)l[e1].x * (double)p[e].x + (double)l[e1].y * (double)p[e].y);
+                vi[g] += s1 * ((double)l[e].x * (double)p[e1].y - (double)l[e].y * (double)p[e1].x) +
+                         ((double)l[e1].x * (double)p[e].y - (double)l[e1].y * (double)p[e].x);
+            }
+        }
+#pragma unroll
+        for (int g = 0; g < G; g++) {
+            const cx<R> th = mk<R>((R)st.thr[g], (R)st.thi[g]), fe = mk<R>((R)st.fer[g], (R)st.fei[g]);
+            const cx<R> f1 = st.y[g] ? mk<R>(-fe.x, -fe.y) : fe;    // factor of the element with the qubit set
+#pragma unroll
+            for (unsigned e = 0; e < E; e++) {
+                if ((e >> g) & 1u) continue;
+                const unsigned e1 = e | (1u << g);
+                const cx<R> p0 = p[e], p1 = p[e1], l0 = l[e], l1 = l[e1];
+                p[e] = cfma(cmul(th, p0), f1, p1);
+                p[e1] = cfma(cmul(th, p1), fe, p0);
+                l[e] = cfma(cmul(th, l0), f1, l1);
+                l[e1] = cfma(cmul(th, l1), fe, l0);
+            }
+        }
+#pragma unroll
+        for (unsigned e = 0; e < E; e++) { adj_st<R, NT>(phi + (x | off[e]), p[e]); adj_st<R, NT>(lam + (x | off[e]), l[e]); }
+    }
+#pragma unroll
+    for (int g = 0; g < G; g++) {
+        double a = vr[g], b = vi[g];
+        block_sum2(a, b);
+        if (threadIdx.x == 0) {
+            double* row = rows + stride * st.slot[g] + 2 * blockIdx.x;
+            row[0] = st.fr[g] * a - st.fi[g] * b;
+            row[1] = st.fr[g] * b + st.fi[g] * a;
+        }
+        __syncthreads();    // block_sum2's wave partials are reused by the next gadget
     }
 }
 
@@ -374,26 +473,53 @@ void launch_pauli_element(hipStream_t s, const cx<R>* a, const cx<R>* b, uint64_
     hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(BLOCK), 0, s, ws, nb, dout);
 }
 
+uint64_t adjoint_row_stride(unsigned nq) { return 2ULL * red_blocks(1ULL << nq); }
+
+void launch_grad_sums(hipStream_t s, const double* rows, uint64_t stride, const uint32_t* nb, unsigned numSlots,
+                      double* res) {
+    if (numSlots) hipLaunchKernelGGL(k_grad_sums, dim3(numSlots), dim3(BLOCK), 0, s, rows, stride, nb, res);
+}
+
 template <typename R>
-void launch_adjoint_step(hipStream_t s, cx<R>* phi, cx<R>* lam, unsigned nq, const AdjointStepDev& st, double* ws,
-                         double* dout) {
+unsigned launch_adjoint_step(hipStream_t s, cx<R>* phi, cx<R>* lam, unsigned nq, const AdjointStepDev& st,
+                             double* rows, uint64_t stride) {
     const uint64_t numIts = st.mxy ? 1ULL << (nq - 1) : 1ULL << nq;
     const unsigned hb = st.mxy ? 63u - (unsigned)__builtin_clzll(st.mxy) : 0u;
     unsigned nb = red_blocks(numIts);
     const uint64_t lineBits = 128 / sizeof(cx<R>) - 1;
     auto* kern = (st.mxy & lineBits) ? k_adjoint_step<R, false> : k_adjoint_step<R, true>;
-    hipLaunchKernelGGL(kern, dim3(nb), dim3(BLOCK), 0, s, phi, lam, numIts, hb, st, ws);
-    hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(BLOCK), 0, s, ws, nb, dout);
+    hipLaunchKernelGGL(kern, dim3(nb), dim3(BLOCK), 0, s, phi, lam, numIts, hb, st, rows, stride);
+    return nb;
 }
 
 template <typename R>
-void launch_adjoint_diag(hipStream_t s, cx<R>* phi, cx<R>* lam, unsigned nq, const AdjointDiagDev& st, double* ws,
-                         double* const* dout) {
+unsigned launch_adjoint_diag(hipStream_t s, cx<R>* phi, cx<R>* lam, unsigned nq, const AdjointDiagDev& st,
+                             double* rows, uint64_t stride) {
     const uint64_t num = 1ULL << nq;
     const unsigned nb = red_blocks(num);
-    hipLaunchKernelGGL(k_adjoint_diag<R>, dim3(nb), dim3(BLOCK), 0, s, phi, lam, num, st, ws);
-    for (unsigned g = 0; g < st.count; g++)
-        hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(BLOCK), 0, s, ws + 2 * (size_t)g * nb, nb, dout[g]);
+    auto* kern = nq > 32 ? k_adjoint_diag<R, true> : k_adjoint_diag<R, false>;
+    hipLaunchKernelGGL(kern, dim3(nb), dim3(BLOCK), 0, s, phi, lam, num, st, rows, stride);
+    return nb;
+}
+
+template <typename R>
+unsigned launch_adjoint_qrun(hipStream_t s, cx<R>* phi, cx<R>* lam, unsigned nq, const AdjointQubitRunDev& st,
+                             double* rows, uint64_t stride) {
+    const unsigned G = st.count;
+    if (G < 2 || G > ADJ_QRUN_MAX || nq < G) fail(FSA_ERR_INTERNAL, "adjoint qubit run of unsupported length");
+    const uint64_t numGroups = 1ULL << (nq - G);
+    const unsigned nb = red_blocks(numGroups);
+    uint64_t bits = 0;
+    for (unsigned g = 0; g < G; g++) bits |= 1ULL << st.q[g];
+    const bool nt = (bits & (128 / sizeof(cx<R>) - 1)) == 0;
+#define FSA_QR(GV)                                                                                                \
+    if (G == GV) {                                                                                                \
+        auto* kern = nt ? k_adjoint_qrun<R, GV, true> : k_adjoint_qrun<R, GV, false>;                             \
+        hipLaunchKernelGGL(kern, dim3(nb), dim3(BLOCK), 0, s, phi, lam, numGroups, st, rows, stride);             \
+    }
+    FSA_QR(2) FSA_QR(3)
+#undef FSA_QR
+    return nb;
 }
 
 // ws must hold 2 * numTerms * ceil(2^nbits / EXP_CHUNK) doubles
@@ -474,10 +600,12 @@ void launch_f32_to_f64(hipStream_t s, const cx<float>* in, cx<double>* out, uint
 #define FSA_INSTANTIATE_RED(R)                                                                               \
     template void launch_norm2<R>(hipStream_t, const cx<R>*, uint64_t, double*, double*);                     \
     template void launch_inner<R>(hipStream_t, const cx<R>*, const cx<R>*, uint64_t, double*, double*);       \
-    template void launch_adjoint_step<R>(hipStream_t, cx<R>*, cx<R>*, unsigned, const AdjointStepDev&, double*,  \
-                                         double*);                                                              \
-    template void launch_adjoint_diag<R>(hipStream_t, cx<R>*, cx<R>*, unsigned, const AdjointDiagDev&, double*,  \
-                                         double* const*);                                                       \
+    template unsigned launch_adjoint_step<R>(hipStream_t, cx<R>*, cx<R>*, unsigned, const AdjointStepDev&,       \
+                                             double*, uint64_t);                                                \
+    template unsigned launch_adjoint_diag<R>(hipStream_t, cx<R>*, cx<R>*, unsigned, const AdjointDiagDev&,       \
+                                             double*, uint64_t);                                                \
+    template unsigned launch_adjoint_qrun<R>(hipStream_t, cx<R>*, cx<R>*, unsigned, const AdjointQubitRunDev&,   \
+                                             double*, uint64_t);                                                \
     template void launch_pauli_element<R>(hipStream_t, const cx<R>*, const cx<R>*, uint64_t, PauliTermDev,     \
                                           double*, double*);                                                  \
     template void launch_dm_expec<R>(hipStream_t, const cx<R>*, unsigned, const PauliTermDev*, unsigned,       \

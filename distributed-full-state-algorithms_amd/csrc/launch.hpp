@@ -81,11 +81,19 @@ struct AdjointStepDev {
     double thr, thi, fer, fei;    // Pauli gadget U^dag: v_j <- th v_j + (+-fe) v_{j ^ mxy}
     double e0r, e0i, e1r, e1i;    // phase gadget U^dag: v_j <- v_j e0 (even parity) or e1 (odd)
     uint32_t phase;               // 1: phase gadget
-    uint32_t pad;
+    uint32_t slot;                // row slot of its partial sums (the gadget's parameter)
 };
+// Partial sums of the adjoint sweep: every launch below writes, for each of
+// its gadgets, one (re, im) row per workgroup at rows + stride * slot + 2 *
+// workgroup, and returns the number of workgroups; launch_grad_sums adds each
+// slot's rows in row order once the sweep is done (one launch for all of them).
+uint64_t adjoint_row_stride(unsigned nq);   // doubles between slots
 template <typename R>
-void launch_adjoint_step(hipStream_t s, cx<R>* phi, cx<R>* lam, unsigned nq, const AdjointStepDev& st, double* ws,
-                         double* dout);
+unsigned launch_adjoint_step(hipStream_t s, cx<R>* phi, cx<R>* lam, unsigned nq, const AdjointStepDev& st,
+                             double* rows, uint64_t stride);
+// res[2 p], res[2 p + 1] = sum of the first nb[p] rows of slot p, p < numSlots
+void launch_grad_sums(hipStream_t s, const double* rows, uint64_t stride, const uint32_t* nb, unsigned numSlots,
+                      double* res);
 // a run of consecutive diagonal gadgets of the reverse sweep in one pass
 // (kernels_reduce.hip): every term <lambda|f_g P_g|phi> is taken from the
 // run's input states (diagonal unitaries leave <D lambda|P|D phi> unchanged
@@ -96,13 +104,30 @@ struct AdjointDiagDev {
     double fr[ADJ_DIAG_MAX], fi[ADJ_DIAG_MAX];        // term factors f_g
     double e0r[ADJ_DIAG_MAX], e0i[ADJ_DIAG_MAX];      // U^dag factor on even parity
     double e1r[ADJ_DIAG_MAX], e1i[ADJ_DIAG_MAX];      // and on odd parity
+    uint32_t slot[ADJ_DIAG_MAX];                      // row slots of the gadgets' partial sums
     uint32_t count;                                   // gadgets in the run (1..ADJ_DIAG_MAX)
     uint32_t pad;
 };
-// ws: ADJ_DIAG_MAX * reduce_workspace_doubles doubles; dout[g]: 2 doubles each
+// a run of 2-3 one-qubit X/Y gadgets on distinct qubits in one pass: each
+// commutes with the others' Pauli operators, so every term is taken from the
+// run's input states; then each gadget's U^dag acts on its pairs
+constexpr unsigned ADJ_QRUN_MAX = 3;
+struct AdjointQubitRunDev {
+    uint32_t q[ADJ_QRUN_MAX];                         // qubit of gadget g (run order)
+    uint32_t y[ADJ_QRUN_MAX];                         // 1: Y on it (signs by its value), 0: X
+    uint32_t sorted[ADJ_QRUN_MAX];                    // the run's qubits ascending
+    uint32_t slot[ADJ_QRUN_MAX];                      // row slots of the gadgets' partial sums
+    uint32_t count;                                   // gadgets in the run (2..ADJ_QRUN_MAX)
+    double fr[ADJ_QRUN_MAX], fi[ADJ_QRUN_MAX];        // term factors f_g = i^{#Y}
+    double thr[ADJ_QRUN_MAX], thi[ADJ_QRUN_MAX];      // U^dag: v_j <- th v_j + (+-fe) v_{j ^ bit}
+    double fer[ADJ_QRUN_MAX], fei[ADJ_QRUN_MAX];
+};
 template <typename R>
-void launch_adjoint_diag(hipStream_t s, cx<R>* phi, cx<R>* lam, unsigned nq, const AdjointDiagDev& st, double* ws,
-                         double* const* dout);
+unsigned launch_adjoint_qrun(hipStream_t s, cx<R>* phi, cx<R>* lam, unsigned nq, const AdjointQubitRunDev& st,
+                             double* rows, uint64_t stride);
+template <typename R>
+unsigned launch_adjoint_diag(hipStream_t s, cx<R>* phi, cx<R>* lam, unsigned nq, const AdjointDiagDev& st,
+                             double* rows, uint64_t stride);
 template <typename R>
 void launch_pauli_element(hipStream_t s, const cx<R>* a, const cx<R>* b, uint64_t num, PauliTermDev term,
                           double* ws, double* dout);

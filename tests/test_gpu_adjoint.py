@@ -153,6 +153,62 @@ def test_adjoint_random_circuits_vs_oracle_shift(fsa, orc, seed):
     assert rel(psi0.download(), psi0h) == 0
 
 
+@pytest.mark.parametrize("n", [9, 12])
+def test_adjoint_gadget_runs_vs_oracle_shift(fsa, orc, n):
+    # the reverse sweep takes runs of consecutive diagonal gadgets (phase
+    # gadgets, Z-only Pauli gadgets) up to eight per pass, and runs of one-qubit
+    # X/Y gadgets on distinct qubits two or three per pass; runs end at other
+    # ops, at a repeated qubit and at the run limits
+    rng = np.random.default_rng(410 + n)
+    u = lambda: np.linalg.svd(rng.normal(size=(2, 2)) + 1j * rng.normal(size=(2, 2)))[0]
+    ang = lambda: float(rng.uniform(0, 6.3))
+    ops = []
+    for layer in range(3):
+        q = [int(x) for x in rng.permutation(n)]
+        for i in range(n - 1):                                         # a run of n - 1 diagonal gadgets
+            if i % 4 == 3:
+                ops.append(("pauliGadget", ([q[i], q[i + 1]], [3, 3], ang())))
+            else:
+                ops.append(("phaseGadget", ([q[i], q[i + 1]], ang())))
+        ops.append(("oneTargGate", (q[0], u())))
+        for grp in (q[:2], q[2:5], q[5:7]):                          # X/Y runs of 2, 3 and 2
+            for x in grp:
+                ops.append(("pauliGadget", ([x], [int(rng.integers(1, 3))], ang())))
+            ops.append(("oneTargGate", (grp[0], u())))
+        ops.append(("pauliGadget", ([q[7]], [1], ang())))              # repeated qubit ends a run
+        ops.append(("pauliGadget", ([q[7]], [2], ang())))
+        ops.append(("pauliGadget", ([q[8]], [2], ang())))
+        for x in q[:5]:                                                # five in a row: runs of 3 and 2
+            ops.append(("pauliGadget", ([x], [1], ang())))
+    T = 8
+    paulis = np.zeros((T, n), dtype=np.uint32)
+    for t in range(T):
+        w = int(rng.integers(1, 4))
+        paulis[t, rng.choice(n, w, replace=False)] = rng.integers(1, 4, w)
+    coeffs = rng.normal(size=T)
+    v = rng.normal(size=1 << n) + 1j * rng.normal(size=1 << n)
+    psi0h = v / np.linalg.norm(v)
+    thetas = np.array([a[-1] for nm, a in ops if nm in ("pauliGadget", "phaseGadget")])
+    shift = np.empty_like(thetas)
+    for k in range(len(thetas)):
+        tp, tm = thetas.copy(), thetas.copy()
+        tp[k] += np.pi / 4
+        tm[k] -= np.pi / 4
+        shift[k] = oracle_value(orc, psi0h, ops, tp, coeffs, paulis) - oracle_value(orc, psi0h, ops, tm, coeffs, paulis)
+    f0 = oracle_value(orc, psi0h, ops, thetas, coeffs, paulis)
+    c = fsa.Circuit(ops)
+    psi0 = fsa.StateVector(n)
+    psi0.upload(psi0h)
+    expec, grads = fsa.adjoint_gradient(c, psi0, coeffs, paulis)
+    assert abs(expec - f0) <= 1e-12 * max(1, abs(f0))
+    assert rel(grads, shift) <= 1e-11, (n, rel(grads, shift))
+    psi32 = fsa.StateVector(n, precision="fp32")
+    psi32.upload(psi0h)
+    e32, g32 = fsa.adjoint_gradient(c, psi32, coeffs, paulis)
+    print(f"gadget runs n={n}: fp64 rel {rel(grads, shift):.2e}, fp32 rel {rel(g32, shift):.2e}")
+    assert rel(g32, shift) <= 1e-5
+
+
 def test_autograd_binding(fsa, circ):
     import torch
 
