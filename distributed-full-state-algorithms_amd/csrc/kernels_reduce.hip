@@ -346,6 +346,62 @@ __global__ __launch_bounds__(BLOCK) void k_adjoint_qrun(cx<R>* __restrict__ phi,
     }
 }
 
+template <typename R>
+__device__ __forceinline__ cx<R> shfl_xor_cx(cx<R> v, int m) {
+    return mk<R>(__shfl_xor(v.x, m, 64), __shfl_xor(v.y, m, 64));
+}
+
+// The same run when every qubit of it is below 6: lane l of a wave holds
+// amplitude base + l of phi and lambda (whole 64-amplitude blocks, so every
+// line is read whole), and the partner j ^ bit comes from lane l ^ bit.  Each
+// element adds conj(lambda_j) s(j') phi_j' (j' = j ^ bit, s(j') = -1 for a Y
+// gadget when j' has the bit set), which over a pair is k_adjoint_step's
+// term; then each gadget in turn maps v_j <- th v_j + s(j') fe v_j'.
+template <typename R, int G>
+__global__ __launch_bounds__(BLOCK) void k_adjoint_qrun_lane(cx<R>* __restrict__ phi, cx<R>* __restrict__ lam,
+                                                             uint64_t num, AdjointQubitRunDev st,
+                                                             double* __restrict__ rows, uint64_t stride) {
+    double vr[G], vi[G];
+#pragma unroll
+    for (int g = 0; g < G; g++) { vr[g] = 0; vi[g] = 0; }
+    // num is a multiple of 64 and so is the stride: a wave's lanes stay together
+    for (uint64_t j = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; j < num; j += (uint64_t)gridDim.x * BLOCK) {
+        cx<R> p = adj_ld<R, true>(phi + j), l = adj_ld<R, true>(lam + j);
+#pragma unroll
+        for (int g = 0; g < G; g++) {
+            const cx<R> pp = shfl_xor_cx(p, 1 << st.q[g]);
+            const bool partnerSet = ((j >> st.q[g]) & 1ULL) == 0;
+            const double sg = (st.y[g] && partnerSet) ? -1.0 : 1.0;
+            vr[g] += sg * ((double)l.x * (double)pp.x + (double)l.y * (double)pp.y);
+            vi[g] += sg * ((double)l.x * (double)pp.y - (double)l.y * (double)pp.x);
+        }
+#pragma unroll
+        for (int g = 0; g < G; g++) {
+            const int m = 1 << st.q[g];
+            const cx<R> pp = shfl_xor_cx(p, m), lp = shfl_xor_cx(l, m);
+            const bool partnerSet = ((j >> st.q[g]) & 1ULL) == 0;
+            const cx<R> th = mk<R>((R)st.thr[g], (R)st.thi[g]);
+            const cx<R> fe = mk<R>((R)st.fer[g], (R)st.fei[g]);
+            const cx<R> f = (st.y[g] && partnerSet) ? mk<R>(-fe.x, -fe.y) : fe;
+            p = cfma(cmul(th, p), f, pp);
+            l = cfma(cmul(th, l), f, lp);
+        }
+        adj_st<R, true>(phi + j, p);
+        adj_st<R, true>(lam + j, l);
+    }
+#pragma unroll
+    for (int g = 0; g < G; g++) {
+        double a = vr[g], b = vi[g];
+        block_sum2(a, b);
+        if (threadIdx.x == 0) {
+            double* row = rows + stride * st.slot[g] + 2 * blockIdx.x;
+            row[0] = st.fr[g] * a - st.fi[g] * b;
+            row[1] = st.fr[g] * b + st.fi[g] * a;
+        }
+        __syncthreads();    // block_sum2's wave partials are reused by the next gadget
+    }
+}
+
 // out[j] = sum_t f_t (-1)^{par(j1 & myz_t)} in[j1], j1 = j ^ mxy_t
 template <typename R>
 __global__ __launch_bounds__(BLOCK) void k_apply_pauli_sum(const cx<R>* __restrict__ in, cx<R>* __restrict__ out,
@@ -508,9 +564,17 @@ unsigned launch_adjoint_qrun(hipStream_t s, cx<R>* phi, cx<R>* lam, unsigned nq,
     const unsigned G = st.count;
     if (G < 2 || G > ADJ_QRUN_MAX || nq < G) fail(FSA_ERR_INTERNAL, "adjoint qubit run of unsupported length");
     const uint64_t numGroups = 1ULL << (nq - G);
-    const unsigned nb = red_blocks(numGroups);
     uint64_t bits = 0;
     for (unsigned g = 0; g < G; g++) bits |= 1ULL << st.q[g];
+    if (bits < 64 && nq >= 6) {
+        // every qubit of the run inside a wave's 64 consecutive amplitudes
+        const uint64_t num = 1ULL << nq;
+        const unsigned nbl = red_blocks(num);
+        if (G == 2) hipLaunchKernelGGL((k_adjoint_qrun_lane<R, 2>), dim3(nbl), dim3(BLOCK), 0, s, phi, lam, num, st, rows, stride);
+        else hipLaunchKernelGGL((k_adjoint_qrun_lane<R, 3>), dim3(nbl), dim3(BLOCK), 0, s, phi, lam, num, st, rows, stride);
+        return nbl;
+    }
+    const unsigned nb = red_blocks(numGroups);
     const bool nt = (bits & (128 / sizeof(cx<R>) - 1)) == 0;
 #define FSA_QR(GV)                                                                                                \
     if (G == GV) {                                                                                                \
