@@ -83,6 +83,23 @@ def test_sv_single_ops(fsa, orc, mode, n):
         assert err <= TOL64, (n, t, name, args[:-1] if name != "swapGate" else args, err)
 
 
+@pytest.mark.parametrize("n", [6, 7, 12])
+def test_sv_one_target_every_qubit_eager(fsa, orc, n):
+    # every target, FP64 and FP32: targets inside a 128-byte line take the
+    # kernel's cached form, the others its non-temporal form (FP64)
+    rng = np.random.default_rng(77 + n)
+    for prec, tol in (("fp64", TOL64), ("fp32", 1e-6)):
+        psi = fsa.StateVector(n, mode=0, precision=prec)
+        for t in range(n):
+            u = np.linalg.svd(rng.normal(size=(2, 2)) + 1j * rng.normal(size=(2, 2)))[0]
+            v = rs(rng, n)
+            psi.upload(v)
+            ref = v.copy()
+            apply_sv(fsa, orc, psi, ref, "oneTargGate", (t, u))
+            err = rel(psi.download(), ref)
+            assert err <= tol, (prec, n, t, err)
+
+
 TUNINGS = {"default": {}, "valu-dense": {"FSA_MFMA": "0"}, "sweeps": {"FSA_SWEEPS": "1"},
            "sweeps-valu": {"FSA_SWEEPS": "1", "FSA_MFMA": "0"}, "tile13": {"FSA_TILE_BITS": "13"},
            "tile13-low5": {"FSA_TILE_BITS": "13", "FSA_TILE_LOW": "5"}}
