@@ -250,6 +250,32 @@ void adjoint_impl(Circuit& c, State& psi0, const double* coeffs, unsigned T, con
     auto qubit_gadget = [](const HOp& o) {
         return o.param >= 0 && o.kind == OP_PAULI && __builtin_popcountll(o.mxy) == 1 && (o.myz & ~o.mxy) == 0;
     };
+    // the diagonal run starting at op j (going backwards): returns its length
+    auto build_diag = [&](size_t j0, AdjointDiagDev& d) {
+        std::memset(&d, 0, sizeof(d));
+        size_t j = j0 + 1;
+        while (j-- > 0 && d.count < ADJ_DIAG_MAX && diag_gadget(c.ops[j])) {
+            const HOp& g = c.ops[j];
+            const HOp gi = inverse(g);
+            const unsigned i = d.count++;
+            cd e0, e1;
+            if (g.kind == OP_PAULI) {              // th v + (+-fe) v, f = i^{#Y} (1: no Y)
+                d.myz[i] = g.myz;
+                d.fr[i] = g.pw.real(); d.fi[i] = g.pw.imag();
+                e0 = gi.th + gi.fe;
+                e1 = gi.th - gi.fe;
+            } else {
+                d.myz[i] = g.pmask;
+                d.fr[i] = 1.0; d.fi[i] = 0.0;
+                e0 = cd(std::cos(gi.theta), std::sin(gi.theta));
+                e1 = cd(std::cos(gi.theta), -std::sin(gi.theta));
+            }
+            d.e0r[i] = e0.real(); d.e0i[i] = e0.imag();
+            d.e1r[i] = e1.real(); d.e1i[i] = e1.imag();
+            d.slot[i] = (uint32_t)g.param;
+        }
+        return d.count;
+    };
     for (size_t k = c.ops.size(); k-- > 0;) {
         const HOp& op = c.ops[k];
         if (qubit_gadget(op) && k > 0 && qubit_gadget(c.ops[k - 1]) && c.ops[k - 1].mxy != op.mxy) {
@@ -271,37 +297,23 @@ void adjoint_impl(Circuit& c, State& psi0, const double* coeffs, unsigned T, con
             }
             std::copy(d.q, d.q + d.count, d.sorted);
             std::sort(d.sorted, d.sorted + d.count);
+            // a run on qubits below 6 takes the diagonal run after it (if any)
+            // into the same pass
+            AdjointDiagDev tail;
+            unsigned nt = 0;
+            if (used < 64 && n >= 6 && k >= d.count && diag_gadget(c.ops[k - d.count]))
+                nt = build_diag(k - d.count, tail);
             ProfScope prof("adjoint_qrun", st);
-            const unsigned nb = launch_adjoint_qrun<R>(st, a_phi, a_lam, n, d, rows, stride);
+            const unsigned nb = launch_adjoint_qrun<R>(st, a_phi, a_lam, n, d, rows, stride, nt ? &tail : nullptr);
             hip_check(hipGetLastError(), "adjoint qubit run launch");
             for (unsigned i = 0; i < d.count; i++) nbs[d.slot[i]] = nb;
-            k -= d.count - 1;       // the loop's k-- moves past the run
+            for (unsigned i = 0; i < nt; i++) nbs[tail.slot[i]] = nb;
+            k -= d.count + nt - 1;  // the loop's k-- moves past the run and its tail
             continue;
         }
         if (diag_gadget(op) && k > 0 && diag_gadget(c.ops[k - 1])) {
             AdjointDiagDev d;
-            std::memset(&d, 0, sizeof(d));
-            size_t j = k + 1;
-            while (j-- > 0 && d.count < ADJ_DIAG_MAX && diag_gadget(c.ops[j])) {
-                const HOp& g = c.ops[j];
-                const HOp gi = inverse(g);
-                const unsigned i = d.count++;
-                cd e0, e1;
-                if (g.kind == OP_PAULI) {              // th v + (+-fe) v, f = i^{#Y} (1: no Y)
-                    d.myz[i] = g.myz;
-                    d.fr[i] = g.pw.real(); d.fi[i] = g.pw.imag();
-                    e0 = gi.th + gi.fe;
-                    e1 = gi.th - gi.fe;
-                } else {
-                    d.myz[i] = g.pmask;
-                    d.fr[i] = 1.0; d.fi[i] = 0.0;
-                    e0 = cd(std::cos(gi.theta), std::sin(gi.theta));
-                    e1 = cd(std::cos(gi.theta), -std::sin(gi.theta));
-                }
-                d.e0r[i] = e0.real(); d.e0i[i] = e0.imag();
-                d.e1r[i] = e1.real(); d.e1i[i] = e1.imag();
-                d.slot[i] = (uint32_t)g.param;
-            }
+            build_diag(k, d);
             ProfScope prof("adjoint_diag", st);
             const unsigned nb = launch_adjoint_diag<R>(st, a_phi, a_lam, n, d, rows, stride);
             hip_check(hipGetLastError(), "adjoint diagonal run launch");

@@ -5,6 +5,8 @@
 // per workgroup (wavefront shuffle, then LDS across the 4 waves), and a single
 // workgroup sums the partials in index order.  Results never depend on timing.
 #include "engine.hpp"
+
+#include <cstring>
 #include "launch.hpp"
 
 namespace fsa {
@@ -357,13 +359,30 @@ __device__ __forceinline__ cx<R> shfl_xor_cx(cx<R> v, int m) {
 // element adds conj(lambda_j) s(j') phi_j' (j' = j ^ bit, s(j') = -1 for a Y
 // gadget when j' has the bit set), which over a pair is k_adjoint_step's
 // term; then each gadget in turn maps v_j <- th v_j + s(j') fe v_j'.
-template <typename R, int G>
+// DG: the diagonal run that follows in the sweep (dg, 1..ADJ_DIAG_MAX gadgets)
+// is applied in the same pass, after the run, as k_adjoint_diag does it: its
+// terms from the updated amplitudes, then its factors from an LDS table.
+template <typename R, int G, bool DG, bool WIDE>
 __global__ __launch_bounds__(BLOCK) void k_adjoint_qrun_lane(cx<R>* __restrict__ phi, cx<R>* __restrict__ lam,
-                                                             uint64_t num, AdjointQubitRunDev st,
+                                                             uint64_t num, AdjointQubitRunDev st, AdjointDiagDev dg,
                                                              double* __restrict__ rows, uint64_t stride) {
-    double vr[G], vi[G];
+    __shared__ cx<R> tab[DG ? 1u << ADJ_DIAG_MAX : 1u];
+    const unsigned cnt = DG ? dg.count : 0u;
+    if constexpr (DG) {
+        for (unsigned t = threadIdx.x; t < (1u << cnt); t += BLOCK) {
+            cx<R> f = mk<R>((R)1, (R)0);
+            for (unsigned g = 0; g < cnt; g++)
+                f = cmul(f, ((t >> g) & 1u) ? mk<R>((R)dg.e1r[g], (R)dg.e1i[g]) : mk<R>((R)dg.e0r[g], (R)dg.e0i[g]));
+            tab[t] = f;
+        }
+        __syncthreads();
+    }
+    constexpr int DM = DG ? (int)ADJ_DIAG_MAX : 1;
+    double vr[G], vi[G], dr[DM], di[DM];
 #pragma unroll
     for (int g = 0; g < G; g++) { vr[g] = 0; vi[g] = 0; }
+#pragma unroll
+    for (int g = 0; g < DM; g++) { dr[g] = 0; di[g] = 0; }
     // num is a multiple of 64 and so is the stride: a wave's lanes stay together
     for (uint64_t j = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; j < num; j += (uint64_t)gridDim.x * BLOCK) {
         cx<R> p = adj_ld<R, true>(phi + j), l = adj_ld<R, true>(lam + j);
@@ -386,6 +405,24 @@ __global__ __launch_bounds__(BLOCK) void k_adjoint_qrun_lane(cx<R>* __restrict__
             p = cfma(cmul(th, p), f, pp);
             l = cfma(cmul(th, l), f, lp);
         }
+        if constexpr (DG) {
+            const double wr = (double)l.x * (double)p.x + (double)l.y * (double)p.y;
+            const double wi = (double)l.x * (double)p.y - (double)l.y * (double)p.x;
+            unsigned pat = 0;
+#pragma unroll
+            for (unsigned g = 0; g < ADJ_DIAG_MAX; g++) {
+                if (g < cnt) {
+                    // WIDE: states of more than 32 qubits (64-bit parities)
+                    const unsigned odd = WIDE ? par64(j & dg.myz[g]) : par32((uint32_t)j & (uint32_t)dg.myz[g]);
+                    pat |= odd << g;
+                    dr[g] += odd ? -wr : wr;
+                    di[g] += odd ? -wi : wi;
+                }
+            }
+            const cx<R> fd = tab[pat];
+            p = cmul(p, fd);
+            l = cmul(l, fd);
+        }
         adj_st<R, true>(phi + j, p);
         adj_st<R, true>(lam + j, l);
     }
@@ -399,6 +436,21 @@ __global__ __launch_bounds__(BLOCK) void k_adjoint_qrun_lane(cx<R>* __restrict__
             row[1] = st.fr[g] * b + st.fi[g] * a;
         }
         __syncthreads();    // block_sum2's wave partials are reused by the next gadget
+    }
+    if constexpr (DG) {
+#pragma unroll
+        for (unsigned g = 0; g < ADJ_DIAG_MAX; g++) {
+            if (g < cnt) {
+                double a = dr[g], b = di[g];
+                block_sum2(a, b);
+                if (threadIdx.x == 0) {
+                    double* row = rows + stride * dg.slot[g] + 2 * blockIdx.x;
+                    row[0] = dg.fr[g] * a - dg.fi[g] * b;
+                    row[1] = dg.fr[g] * b + dg.fi[g] * a;
+                }
+                __syncthreads();
+            }
+        }
     }
 }
 
@@ -560,7 +612,7 @@ unsigned launch_adjoint_diag(hipStream_t s, cx<R>* phi, cx<R>* lam, unsigned nq,
 
 template <typename R>
 unsigned launch_adjoint_qrun(hipStream_t s, cx<R>* phi, cx<R>* lam, unsigned nq, const AdjointQubitRunDev& st,
-                             double* rows, uint64_t stride) {
+                             double* rows, uint64_t stride, const AdjointDiagDev* tail) {
     const unsigned G = st.count;
     if (G < 2 || G > ADJ_QRUN_MAX || nq < G) fail(FSA_ERR_INTERNAL, "adjoint qubit run of unsupported length");
     const uint64_t numGroups = 1ULL << (nq - G);
@@ -570,10 +622,26 @@ unsigned launch_adjoint_qrun(hipStream_t s, cx<R>* phi, cx<R>* lam, unsigned nq,
         // every qubit of the run inside a wave's 64 consecutive amplitudes
         const uint64_t num = 1ULL << nq;
         const unsigned nbl = red_blocks(num);
-        if (G == 2) hipLaunchKernelGGL((k_adjoint_qrun_lane<R, 2>), dim3(nbl), dim3(BLOCK), 0, s, phi, lam, num, st, rows, stride);
-        else hipLaunchKernelGGL((k_adjoint_qrun_lane<R, 3>), dim3(nbl), dim3(BLOCK), 0, s, phi, lam, num, st, rows, stride);
+        AdjointDiagDev dg;
+        std::memset(&dg, 0, sizeof(dg));
+        if (tail) dg = *tail;
+#define FSA_QL(GV, DGV, WV)                                                                                       \
+        hipLaunchKernelGGL((k_adjoint_qrun_lane<R, GV, DGV, WV>), dim3(nbl), dim3(BLOCK), 0, s, phi, lam, num, st, \
+                           dg, rows, stride)
+        const bool wide = nq > 32;
+        if (G == 2) {
+            if (!tail) FSA_QL(2, false, false);
+            else if (wide) FSA_QL(2, true, true);
+            else FSA_QL(2, true, false);
+        } else {
+            if (!tail) FSA_QL(3, false, false);
+            else if (wide) FSA_QL(3, true, true);
+            else FSA_QL(3, true, false);
+        }
+#undef FSA_QL
         return nbl;
     }
+    if (tail) fail(FSA_ERR_INTERNAL, "a diagonal tail needs a qubit run below qubit 6");
     const unsigned nb = red_blocks(numGroups);
     const bool nt = (bits & (128 / sizeof(cx<R>) - 1)) == 0;
 #define FSA_QR(GV)                                                                                                \
@@ -669,7 +737,7 @@ void launch_f32_to_f64(hipStream_t s, const cx<float>* in, cx<double>* out, uint
     template unsigned launch_adjoint_diag<R>(hipStream_t, cx<R>*, cx<R>*, unsigned, const AdjointDiagDev&,       \
                                              double*, uint64_t);                                                \
     template unsigned launch_adjoint_qrun<R>(hipStream_t, cx<R>*, cx<R>*, unsigned, const AdjointQubitRunDev&,   \
-                                             double*, uint64_t);                                                \
+                                             double*, uint64_t, const AdjointDiagDev*);                         \
     template void launch_pauli_element<R>(hipStream_t, const cx<R>*, const cx<R>*, uint64_t, PauliTermDev,     \
                                           double*, double*);                                                  \
     template void launch_dm_expec<R>(hipStream_t, const cx<R>*, unsigned, const PauliTermDev*, unsigned,       \

@@ -122,9 +122,11 @@ struct AdjointQubitRunDev {
     double thr[ADJ_QRUN_MAX], thi[ADJ_QRUN_MAX];      // U^dag: v_j <- th v_j + (+-fe) v_{j ^ bit}
     double fer[ADJ_QRUN_MAX], fei[ADJ_QRUN_MAX];
 };
+// tail: a diagonal run that follows in the sweep, applied in the same pass
+// (only for runs whose qubits are all below 6; nullptr for none)
 template <typename R>
 unsigned launch_adjoint_qrun(hipStream_t s, cx<R>* phi, cx<R>* lam, unsigned nq, const AdjointQubitRunDev& st,
-                             double* rows, uint64_t stride);
+                             double* rows, uint64_t stride, const AdjointDiagDev* tail = nullptr);
 template <typename R>
 unsigned launch_adjoint_diag(hipStream_t s, cx<R>* phi, cx<R>* lam, unsigned nq, const AdjointDiagDev& st,
                              double* rows, uint64_t stride);
