@@ -2,6 +2,8 @@
 //
 //   emu_drive circuit <file>              fused circuit (make_plan + execute: k_tile, streaming kernels)
 //   emu_drive batch <file> TB BITS LOW P  batch (fsa_batch_* + k_batch), P = 0 FP64 / 1 FP32
+//   emu_drive adjoint <file> P            fsa_adjoint_gradient from |0..0> (fused forward pass,
+//                                         reverse-sweep kernels of kernels_reduce.hip)
 //
 // Input files are written by tests/test_emu_kernels.py.  Output: one line with
 // the errors against the expected values in the file; exit status 0 unless
@@ -137,12 +139,59 @@ int run_batch(const char* path, int tb, int bits, int low, int prec) {
     return 0;
 }
 
+// one job in the batch record format: n, L, T, ops, coeffs, paulis, <H> and gradients
+int run_adjoint(const char* path, int prec) {
+    Reader r{std::fopen(path, "rb")};
+    if (!r.f) return 3;
+    const unsigned n = r.one<uint32_t>(), L = r.one<uint32_t>(), T = r.one<uint32_t>();
+    fsa_circuit* c;
+    fsa_circuit_create(&c);
+    unsigned g = 0;
+    for (unsigned i = 0; i < L; i++)
+        if (add_op(r, c, &g)) { std::printf("add failed: %s\n", fsa_last_error()); return 2; }
+    std::vector<double> coeffs(T), want(1 + g);
+    std::vector<uint32_t> paulis((size_t)T * n);
+    r.get(coeffs.data(), T);
+    r.get(paulis.data(), paulis.size());
+    r.get(want.data(), want.size());
+    std::fclose(r.f);
+    fsa_state st;
+    State& s = st.s;
+    s.N = n;
+    s.nq = n;
+    s.prec = prec;
+    s.mode = FSA_EXEC_FUSED;
+    hipMalloc(&s.amps, s.num() * s.amp_bytes());     // zero-filled
+    if (prec == FSA_FP64) static_cast<cx<double>*>(s.amps)[0] = mk<double>(1, 0);
+    else static_cast<cx<float>*>(s.amps)[0] = mk<float>(1, 0);
+    std::vector<double> gr(g);
+    double e = 0;
+    profile_enable(true);
+    if (fsa_adjoint_gradient(c, &st, coeffs.data(), T, paulis.data(), &e, gr.data())) {
+        std::printf("adjoint: %s\n", fsa_last_error());
+        return 2;
+    }
+    // launches per kernel family, e.g. adjoint_qrun=2
+    for (const ProfEntry& p : profile_read()) std::printf("%s=%llu ", p.name.c_str(), (unsigned long long)p.count);
+    profile_enable(false);
+    double num = 0, den = 0;
+    for (unsigned p = 0; p < g; p++) {
+        num += (gr[p] - want[1 + p]) * (gr[p] - want[1 + p]);
+        den += want[1 + p] * want[1 + p];
+    }
+    std::printf("params=%u expec_abs_err=%.3e grad_rel_err=%.3e\n", g, std::fabs(e - want[0]), std::sqrt(num / den));
+    hipFree(s.amps);
+    fsa_circuit_destroy(c);
+    return 0;
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
     if (argc >= 3 && std::string(argv[1]) == "circuit") return run_circuit(argv[2]);
     if (argc >= 7 && std::string(argv[1]) == "batch")
         return run_batch(argv[2], std::atoi(argv[3]), std::atoi(argv[4]), std::atoi(argv[5]), std::atoi(argv[6]));
-    std::fprintf(stderr, "usage: emu_drive circuit FILE | batch FILE TB BITS LOW PREC\n");
+    if (argc >= 4 && std::string(argv[1]) == "adjoint") return run_adjoint(argv[2], std::atoi(argv[3]));
+    std::fprintf(stderr, "usage: emu_drive circuit FILE | batch FILE TB BITS LOW PREC | adjoint FILE PREC\n");
     return 1;
 }

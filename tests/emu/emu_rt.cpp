@@ -22,6 +22,16 @@ double emu_shfl_xor_d(double v, int mask) {
     return r;
 }
 
+// lanes past the wave's end keep their own value (as on the GPU)
+double emu_shfl_down_d(double v, unsigned off) {
+    const unsigned t = threadIdx.x, w = t / 64, l = t % 64;
+    g_x[w][l] = v;
+    g_wave_bar[w]->arrive_and_wait();
+    const double r = l + off < 64 ? g_x[w][l + off] : v;
+    g_wave_bar[w]->arrive_and_wait();
+    return r;
+}
+
 void emu_wave_gather(double a, double b, double* A, double* B) {
     const unsigned t = threadIdx.x, w = t / 64, l = t % 64;
     g_ga[w][l] = a;

@@ -42,10 +42,13 @@ extern EmuIdx blockIdx, gridDim, blockDim;
 
 void emu_syncthreads();
 double emu_shfl_xor_d(double v, int mask);
+double emu_shfl_down_d(double v, unsigned off);
 void emu_wave_gather(double a, double b, double* A, double* B);   // all 64 lanes' a and b
 inline void __syncthreads() { emu_syncthreads(); }
 template <class T>
 inline T __shfl_xor(T v, int mask, int = 64) { return (T)emu_shfl_xor_d((double)v, mask); }
+template <class T>
+inline T __shfl_down(T v, unsigned off, int = 64) { return (T)emu_shfl_down_d((double)v, off); }
 inline unsigned atomicAdd(unsigned* p, unsigned v) { return __atomic_fetch_add(p, v, __ATOMIC_SEQ_CST); }
 
 // v_mfma_*_16x16x4 (layouts measured on gfx950): lane l holds A[l%16][l/16] and

@@ -17,7 +17,10 @@ compared with the oracle:
     tile is written back;
   * a mixed random circuit on 2^13-amplitude tiles;
   * batched <H> and adjoint gradients, LDS-resident and streamed jobs, FP64 and
-    FP32, against an adjoint computed with the oracle.
+    FP32, against an adjoint computed with the oracle;
+  * the single-state adjoint sweep (kernels_reduce.hip): runs of one-qubit X/Y
+    gadgets (group and lane forms, a diagonal tail), runs of Z-only gadgets and
+    single steps, FP64 and FP32, against the same oracle adjoint.
 
 Skipped when the ROCm clang++ is not installed.
 """
@@ -44,7 +47,8 @@ CODES = {"oneTargGate": 0, "manyCtrlOneTargGate": 1, "swapGate": 2, "manyTargGat
 def _build():
     BUILD.mkdir(exist_ok=True)
     exe = BUILD / "emu_drive"
-    sources = [CSRC / f for f in ("kernels_tile.hip", "kernels_stream.hip", "kernels_batch.hip", "engine.cpp",
+    sources = [CSRC / f for f in ("kernels_tile.hip", "kernels_stream.hip", "kernels_batch.hip", "kernels_reduce.hip",
+                                  "engine.cpp",
                                   "circuit.cpp", "batch.cpp", "engine.hpp", "tile_ops.hpp", "common.hpp", "launch.hpp")]
     sources += [EMU / "emu_rt.cpp", EMU / "emu_drive.cpp", EMU / "hip" / "hip_runtime.h"]
     if exe.exists() and all(s.stat().st_mtime < exe.stat().st_mtime for s in sources):
@@ -56,7 +60,7 @@ def _build():
             "static unsigned char smem_raw[262144] __attribute__((aligned(16)));")
         (BUILD / f"{name}_emu.cpp").write_text(src)
     units = [(BUILD / "kernels_tile_emu.cpp", []), (BUILD / "kernels_stream_emu.cpp", []),
-             (CSRC / "kernels_batch.hip", ["-x", "c++"]),
+             (CSRC / "kernels_batch.hip", ["-x", "c++"]), (CSRC / "kernels_reduce.hip", ["-x", "c++"]),
              (CSRC / "engine.cpp", []), (CSRC / "circuit.cpp", []), (CSRC / "batch.cpp", []),
              (EMU / "emu_rt.cpp", []), (EMU / "emu_drive.cpp", [])]
     procs, objs = [], []
@@ -236,6 +240,46 @@ def _oracle_adjoint(orc, n, ops, coeffs, paulis):
         getattr(orc, "sv_" + name)(phi, *inv)
         getattr(orc, "sv_" + name)(lam, *inv)
     return expec, grads[::-1]
+
+
+@pytest.mark.parametrize("prec,tol", [(0, 1e-12), (1, 2e-5)])
+def test_adjoint_runs_on_cpu(drive, orc, tmp_path, prec, tol):
+    # fsa_adjoint_gradient on 9 qubits: per layer a run of 10 Z-only gadgets, then
+    # X/Y gadgets on qubits 0..8.  In the reverse sweep: qubits 8..6 take the
+    # group kernel, 5..3 the lane kernel, 2..0 the lane kernel with the Z run's
+    # first 8 gadgets as its tail, the rest of the Z run the diagonal kernel;
+    # an X (x) Z gadget and the one-qubit gates take single steps
+    rng = np.random.default_rng(52)
+    n = 9
+    ang = lambda: float(rng.uniform(0, 6.3))
+    ops = []
+    for layer in range(2):
+        for i in range(n - 1):
+            ops.append(("pauliGadget", ([i, i + 1], [3, 3], ang())))
+        ops.append(("pauliGadget", ([0, 4, 8], [3, 3, 3], ang())))
+        ops.append(("pauliGadget", ([1, 5], [3, 3], ang())))
+        for q in range(n):
+            ops.append(("pauliGadget", ([q], [1 if q % 3 else 2], ang())))
+        ops.append(("pauliGadget", ([2, 7], [1, 3], ang())))
+        ops.append(("oneTargGate", (4, _unitary(rng, 2))))
+    T = 6
+    paulis = np.zeros((T, n), np.uint32)
+    for t in range(T):
+        w = int(rng.integers(1, 4))
+        paulis[t, rng.choice(n, w, replace=False)] = rng.integers(1, 4, w)
+    coeffs = rng.normal(size=T)
+    expec, grads = _oracle_adjoint(orc, n, ops, coeffs, paulis)
+    path = tmp_path / "adjoint.bin"
+    with open(path, "wb") as f:
+        np.array([n, len(ops), T], np.uint32).tofile(f)
+        for name, args in ops:
+            _write_op(f, name, args)
+        np.asarray(coeffs, np.float64).tofile(f)
+        np.asarray(paulis, np.uint32).tofile(f)
+        np.array([expec] + grads, np.float64).tofile(f)
+    out = _run(drive, ["adjoint", path, prec])
+    assert out["expec_abs_err"] <= tol and out["grad_rel_err"] <= tol, out
+    assert out["adjoint_qrun"] == 6 and out["adjoint_diag"] == 2 and out["adjoint_step"] == 2, out
 
 
 @pytest.mark.parametrize("prec,tol", [(0, 1e-12), (1, 2e-5)])
