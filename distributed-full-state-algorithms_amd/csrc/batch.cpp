@@ -25,7 +25,7 @@ template <typename R>
 int batch_occupancy(int tb, int m);
 template <typename R>
 bool launch_batch(hipStream_t s, int tb, int m, unsigned grid, const BatchJob* jobs, unsigned numJobs,
-                  const BatchStage* stages, const TileOp* items, const cx<R>* mats, const PauliTermDev* terms,
+                  const BatchStage* stages, const TileOp* items, const cx<double>* mats, const PauliTermDev* terms,
                   const double* params, cx<R>* arena, double* res, unsigned* counter);
 
 constexpr unsigned MAX_STAGE_GRADS_HOST = 512;   // LDS gradient accumulators per pass (kernels_batch.hip)
@@ -124,7 +124,7 @@ struct BlobBuilder {
     std::vector<BatchJob> jobs;
     std::vector<BatchStage> stages;
     std::vector<TileOp> items;
-    std::vector<cx<R>> mats;
+    std::vector<cx<double>> mats;      // FP64 in either precision: amplitudes combine in FP64
     std::vector<PauliTermDev> terms;
 
     // ops[] with per-op angle sign (+1 forward, -1 inverse; 0 for GRAD)
@@ -148,7 +148,7 @@ struct BlobBuilder {
             for (size_t ii = 0; ii < s.ops.size(); ii++) {
                 const int oi = s.ops[ii];
                 const HOp& op = ops[oi];
-                TileOp t = translate_item<R>(op, pos, bs.tileMask, mats);
+                TileOp t = translate_item<double>(op, pos, bs.tileMask, mats);
                 if (op.kind == OP_GRAD && ngrad == MAX_STAGE_GRADS_HOST) {
                     // close the pass and continue the same ops in a new one on the same tiles
                     bs.nitems = (uint32_t)(items.size() - bs.item0);
@@ -232,6 +232,7 @@ void build_impl(Batch& b) {
         J.out = j;
         J.slot0 = numJobs + e.pbase;
         J.nslot = e.numParams;
+        J.nrm = 2 * (numJobs + b.totalParams) + j;
         if (!resident) {
             J.phi = arenaElems;
             J.lam = arenaElems + (1ULL << e.n);
@@ -256,20 +257,20 @@ void build_impl(Batch& b) {
     b.offStages = align(sorted.size() * sizeof(BatchJob));
     b.offItems = align(b.offStages + bb.stages.size() * sizeof(BatchStage));
     b.offMats = align(b.offItems + bb.items.size() * sizeof(TileOp));
-    b.offTerms = align(b.offMats + bb.mats.size() * sizeof(cx<R>));
+    b.offTerms = align(b.offMats + bb.mats.size() * sizeof(cx<double>));
     const size_t blobBytes = align(b.offTerms + bb.terms.size() * sizeof(PauliTermDev));
     std::vector<char> host(blobBytes, 0);
     std::memcpy(host.data() + b.offJobs, sorted.data(), sorted.size() * sizeof(BatchJob));
     std::memcpy(host.data() + b.offStages, bb.stages.data(), bb.stages.size() * sizeof(BatchStage));
     std::memcpy(host.data() + b.offItems, bb.items.data(), bb.items.size() * sizeof(TileOp));
-    if (!bb.mats.empty()) std::memcpy(host.data() + b.offMats, bb.mats.data(), bb.mats.size() * sizeof(cx<R>));
+    if (!bb.mats.empty()) std::memcpy(host.data() + b.offMats, bb.mats.data(), bb.mats.size() * sizeof(cx<double>));
     if (!bb.terms.empty())
         std::memcpy(host.data() + b.offTerms, bb.terms.data(), bb.terms.size() * sizeof(PauliTermDev));
 
     const size_t ampBytes = sizeof(cx<R>);
     dev_alloc(&b.arena, arenaElems * ampBytes, "the batch arena");
     dev_alloc((void**)&b.blob, blobBytes, "the batch plans");
-    b.resDoubles = 2ULL * (numJobs + b.totalParams);
+    b.resDoubles = 2ULL * (numJobs + b.totalParams) + numJobs;   // <H>, gradients, <phi|phi>
     dev_alloc((void**)&b.dres, b.resDoubles * sizeof(double), "the batch results");
     dev_alloc((void**)&b.dparams, std::max<size_t>(1, b.totalParams) * sizeof(double), "the batch angles");
     dev_alloc((void**)&b.dcounter, 256, "the batch counter");
@@ -306,7 +307,7 @@ void run_impl(Batch& b, double* expecs, double* grads) {
                                         reinterpret_cast<const BatchJob*>(b.blob + b.offJobs), numJobs,
                                         reinterpret_cast<const BatchStage*>(b.blob + b.offStages),
                                         reinterpret_cast<const TileOp*>(b.blob + b.offItems),
-                                        reinterpret_cast<const cx<R>*>(b.blob + b.offMats),
+                                        reinterpret_cast<const cx<double>*>(b.blob + b.offMats),
                                         reinterpret_cast<const PauliTermDev*>(b.blob + b.offTerms), b.dparams,
                                         static_cast<cx<R>*>(b.arena), b.dres, b.dcounter);
         hipGraph_t g = nullptr;
@@ -323,10 +324,19 @@ void run_impl(Batch& b, double* expecs, double* grads) {
     hip_check(hipMemcpyAsync(b.hres, b.dres, b.resDoubles * sizeof(double), hipMemcpyDeviceToHost, b.stream),
               "hipMemcpyAsync(results)");
     hip_check(hipStreamSynchronize(b.stream), "hipStreamSynchronize");
-    if (expecs)
-        for (unsigned j = 0; j < numJobs; j++) expecs[j] = b.hres[2 * j];
-    if (grads)
-        for (unsigned p = 0; p < b.totalParams; p++) grads[p] = -2.0 * b.hres[2 * (numJobs + p) + 1];
+    // FP32 renormalisation: the forward state of job j has <phi|phi> = 1 up to
+    // rounding; <H> and every gradient term are quadratic in the state, so
+    // dividing them by the FP64 norm <phi|phi> gives the values of the
+    // normalised state
+    const double* n2 = b.hres + 2 * (numJobs + b.totalParams);
+    for (unsigned j = 0; j < numJobs; j++) {
+        const BatchEntry& e = b.entries[j];
+        const double sc = b.prec == FSA_FP32 ? 1.0 / n2[j] : 1.0;
+        if (expecs) expecs[j] = b.hres[2 * j] * sc;
+        if (grads)
+            for (unsigned p = e.pbase; p < e.pbase + e.numParams; p++)
+                grads[p] = -2.0 * b.hres[2 * (numJobs + p) + 1] * sc;
+    }
 }
 
 void run(Batch& b, double* expecs, double* grads) {

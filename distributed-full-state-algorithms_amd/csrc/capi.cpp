@@ -355,6 +355,14 @@ int fsa_state_copy(fsa_state* dst, fsa_state* src) {
         hip_check(hipStreamSynchronize(s.stream), "hipStreamSynchronize");
         hip_check(hipStreamSynchronize(d.stream), "hipStreamSynchronize");
         hip_check(hipMemcpyAsync(d.amps, s.amps, s.num() * s.amp_bytes(), hipMemcpyDeviceToDevice, d.stream), "hipMemcpyAsync(copy)");
+        // later work on src (its own stream) must not overwrite the source
+        // while the copy still reads it
+        hipEvent_t done = nullptr;
+        hip_check(hipEventCreateWithFlags(&done, hipEventDisableTiming), "hipEventCreate");
+        hipError_t e = hipEventRecord(done, d.stream);
+        if (e == hipSuccess) e = hipStreamWaitEvent(s.stream, done, 0);
+        (void)hipEventDestroy(done);
+        hip_check(e, "ordering the copy before later work on the source");
     });
 }
 

@@ -17,15 +17,6 @@ using namespace fsa;
 
 namespace fsa {
 
-template <typename R>
-int rev_tile_bits();
-template <typename R>
-unsigned rev_grid(uint64_t numTiles);
-template <typename R>
-void launch_rev_pass(hipStream_t s, unsigned grid, cx<R>* phi, cx<R>* lam, unsigned n, const BatchStage& st,
-                     const TileOp* items, const cx<R>* mats, double* part, unsigned ngrad, const uint32_t* slots,
-                     double* res);
-
 HOp inverse(const HOp& op) {
     HOp v = op;
     switch (op.kind) {
@@ -46,72 +37,6 @@ HOp inverse(const HOp& op) {
 }
 
 
-template <typename R>
-RevPlan<R> build_reverse_plan(const std::vector<HOp>& ops, unsigned n) {
-    RevPlan<R> p;
-    for (size_t k = ops.size(); k-- > 0;) {
-        const HOp& op = ops[k];
-        if (op.param >= 0) {
-            HOp g;
-            g.kind = OP_GRAD;
-            if (op.kind == OP_PAULI) { g.mxy = op.mxy; g.myz = op.myz; g.pw = op.pw; }
-            else { g.mxy = 0; g.myz = op.pmask; g.pw = 1.0; }
-            g.param = op.param;
-            finish_op(g);
-            p.rev.push_back(g);
-        }
-        p.rev.push_back(inverse(op));
-    }
-    SchedParams sp;
-    sp.m = std::min(n, (unsigned)rev_tile_bits<R>());
-    sp.low = n <= sp.m ? n : std::min(5u, sp.m);
-    p.m = sp.m;
-    const std::vector<Stage> stages = schedule(p.rev, n, sp);
-    for (const Stage& stg : stages) {
-        typename RevPlan<R>::Pass r{};
-        r.tiled = stg.tq.size() == sp.m;
-        if (!r.tiled) {
-            r.ops = stg.ops;
-            p.passes.push_back(r);
-            continue;
-        }
-        int pos[64];
-        for (int& x : pos) x = -1;
-        for (size_t i = 0; i < stg.tq.size(); i++) {
-            pos[stg.tq[i]] = (int)i;
-            r.bs.tileMask |= 1ULL << stg.tq[i];
-        }
-        r.bs.m = sp.m;
-        r.bs.item0 = (uint32_t)p.items.size();
-        r.slot0 = p.slots.size();
-        auto close = [&]() {
-            fuse_diag_runs(p.items, r.bs.item0, true);
-            r.bs.nitems = (uint32_t)(p.items.size() - r.bs.item0);
-            p.maxGrad = std::max(p.maxGrad, r.ngrad);
-            p.passes.push_back(r);
-        };
-        for (int oi : stg.ops) {
-            TileOp t = translate_item<R>(p.rev[oi], pos, r.bs.tileMask, p.mats);
-            t.param = NO_PARAM;                          // factors are fixed for this call
-            if (p.rev[oi].kind == OP_GRAD && r.ngrad == REV_MAX_GRADS) {
-                // close the pass and continue the same ops in a new one on the same tiles
-                close();
-                r.bs.item0 = (uint32_t)p.items.size();
-                r.slot0 = p.slots.size();
-                r.ngrad = 0;
-            }
-            if (p.rev[oi].kind == OP_GRAD) {
-                t.p[0] = r.ngrad++;
-                p.slots.push_back(1u + (uint32_t)p.rev[oi].param);
-            }
-            p.items.push_back(t);
-        }
-        close();
-    }
-    return p;
-}
-template RevPlan<double> build_reverse_plan<double>(const std::vector<HOp>&, unsigned);
-template RevPlan<float> build_reverse_plan<float>(const std::vector<HOp>&, unsigned);
 }  // namespace fsa
 
 namespace {
@@ -202,9 +127,26 @@ void adjoint_impl(Circuit& c, State& psi0, const double* coeffs, unsigned T, con
     hipStream_t st = psi0.stream;
     flush(psi0);
     Scratch phi(n, psi0.prec, st), lam(n, psi0.prec, st);
+    // results: slot 0 = <phi|lambda>, slot 1 + p = <lambda|P_p|phi>, then the
+    // FP64 norms <psi0|psi0> and <phi|phi> (FP32 renormalisation), then the
+    // gradient rows of the sweep
+    const unsigned P = c.numParams;
+    const size_t red = reduce_workspace_doubles(phi.s.num());
+    const uint64_t stride = adjoint_row_stride(n);
+    double* ws = phi.s.workspace(red + 2 * (P + 1) + 2 + stride * P + 8);
+    double* res = ws + red;
+    double* norms = res + 2 * (P + 1);
+    double* rows = norms + 2;
+    const bool renorm = psi0.prec == FSA_FP32;
+    if (renorm) launch_norm2<R>(st, (const cx<R>*)psi0.amps, psi0.num(), ws, norms);
     hip_check(hipMemcpyAsync(phi.s.amps, psi0.amps, psi0.num() * psi0.amp_bytes(), hipMemcpyDeviceToDevice, st),
               "hipMemcpyAsync");
     execute(phi.s, plan_for(c, n, psi0.prec));
+    // a unitary circuit keeps the norm; FP32 rounding does not quite.  <H> and
+    // every gradient term are quadratic in phi (lambda = H phi), so the
+    // forward drift is undone by scaling them by <psi0|psi0> / <phi|phi> at
+    // the end instead of rescaling the states
+    if (renorm) launch_norm2<R>(st, (const cx<R>*)phi.s.amps, phi.s.num(), ws, norms + 1);
 
     std::vector<PauliTermDev> terms(T);
     for (unsigned t = 0; t < T; t++) {
@@ -224,22 +166,14 @@ void adjoint_impl(Circuit& c, State& psi0, const double* coeffs, unsigned T, con
     auto* a_lam = (cx<R>*)lam.s.amps;
     launch_apply_pauli_sum<R>(st, a_phi, a_lam, n, dt, T);
 
-    // results: slot 0 = <phi|lambda>, slot 1 + p = <lambda|P_p|phi>
-    const unsigned P = c.numParams;
 
     // Reverse sweep, op by op: for a gadget, <lambda|P_k|phi> and U_k^dag on phi
     // and on lambda in one pass over both states (k_adjoint_step); any other op
-    // is undone on each state by its own kernel.  (The tile-pass form of this
-    // sweep, build_reverse_plan + launch_rev_pass, is not used by this entry point.)
+    // is undone on each state by its own kernel.
     // Runs of consecutive parametrised diagonal gadgets (phase gadgets, Z-only
     // Pauli gadgets) take one pass per ADJ_DIAG_MAX of them (k_adjoint_diag).
     // Every gadget's partial sums go to the row slot of its parameter; one
     // launch adds each slot's rows after the sweep (launch_grad_sums).
-    const size_t red = reduce_workspace_doubles(phi.s.num());
-    const uint64_t stride = adjoint_row_stride(n);
-    double* ws = phi.s.workspace(red + 2 * (P + 1) + stride * P + 8);
-    double* res = ws + red;
-    double* rows = res + 2 * (P + 1);
     std::vector<uint32_t> nbs(P, 0);
     launch_inner<R>(st, a_phi, a_lam, phi.s.num(), ws, res);
     auto diag_gadget = [](const HOp& o) {
@@ -352,11 +286,12 @@ void adjoint_impl(Circuit& c, State& psi0, const double* coeffs, unsigned T, con
         launch_grad_sums(st, rows, stride, dnb, P, res + 2);
         hip_check(hipGetLastError(), "gradient sums launch");
     }
-    std::vector<double> host(2 * (P + 1));
+    std::vector<double> host(2 * (P + 1) + 2);
     hip_check(hipMemcpyAsync(host.data(), res, host.size() * sizeof(double), hipMemcpyDeviceToHost, st), "hipMemcpyAsync");
     hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
-    *expec = host[0];
-    for (unsigned p = 0; p < P; p++) grads[p] = -2.0 * host[2 * (1 + p) + 1];
+    const double sc = renorm && host[2 * (P + 1) + 1] > 0 ? host[2 * (P + 1)] / host[2 * (P + 1) + 1] : 1.0;
+    *expec = host[0] * sc;
+    for (unsigned p = 0; p < P; p++) grads[p] = -2.0 * host[2 * (1 + p) + 1] * sc;
 }
 
 }  // namespace

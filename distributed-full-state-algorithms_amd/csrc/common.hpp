@@ -114,7 +114,6 @@ enum OpKind : uint32_t {
     OP_DAMP = 10,
     OP_SWEEP = 11,   // tile pass item: a run of ops applied to register groups (see kernels_tile.hip)
     OP_GRAD = 12,    // batched adjoint: accumulate <lambda|P|phi> over the tile (see kernels_batch.hip)
-    OP_DIAG_RUN = 13,// header of k consecutive diagonal items applied in one LDS sweep (tile_ops.hpp)
     OP_DIAG4 = 14,   // diagonal factor f[(parity1 << 1) | parity2] (pass-end item applied at the write-back)
 };
 
@@ -151,8 +150,6 @@ constexpr uint32_t MAT_UNPLACED = 0xffffffffu;  // MFMA-form MATK item not yet g
 constexpr uint32_t tile_mat_lds_bytes(size_t tileBytes) {
     return tileBytes > 65536 ? (uint32_t)(163840 - tileBytes) : 16384u;
 }
-constexpr uint32_t DIAG_RUN_MAX = 32;      // items per OP_DIAG_RUN
-constexpr uint32_t DIAG_RUN_GRADS = 6;     // gradient items per OP_DIAG_RUN
 constexpr int TILE_MAX_BITS = 14;
 
 // Geometry of one fused pass: the sorted tile qubits (global positions).
@@ -184,7 +181,7 @@ struct BatchJob {
     uint32_t term0, nterm;// observable terms (PauliTermDev)
     uint32_t out;         // result slot of <H> (two doubles)
     uint32_t slot0, nslot;// result slots of the job's gradients, zeroed at the start
-    uint32_t pad;
+    uint32_t nrm;         // result double that receives <phi|phi> after the forward plan
 };
 
 }  // namespace fsa

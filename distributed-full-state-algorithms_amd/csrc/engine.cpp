@@ -216,27 +216,50 @@ Plan make_plan(const std::vector<HOp>& ops, unsigned nq, int precision) {
 // ---- parameter ring ----------------------------------------------------------
 
 ParamRing::~ParamRing() {
+    // the owner has synchronised the stream before destroying the ring
+    for (auto& b : retired) {
+        (void)hipHostFree(b.first);
+        (void)hipFree(b.second);
+    }
     if (host) (void)hipHostFree(host);
     if (dev) (void)hipFree(dev);
 }
 
+void ParamRing::release_retired() {
+    for (auto& b : retired) {
+        hip_check(hipHostFree(b.first), "hipHostFree");
+        hip_check(hipFree(b.second), "hipFree");
+    }
+    retired.clear();
+}
+
 void* ParamRing::push(const void* data, size_t bytes) {
     size_t need = (bytes + 255) & ~(size_t)255;
-    if (need > cap) {
-        if (stream) hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
-        if (host) hip_check(hipHostFree(host), "hipHostFree");
-        if (dev) hip_check(hipFree(dev), "hipFree");
-        host = nullptr;
-        dev = nullptr;
-        size_t ncap = std::max<size_t>({need, 2 * cap, (size_t)1 << 20});
-        hip_check(hipHostMalloc((void**)&host, ncap, hipHostMallocDefault), "hipHostMalloc(param ring)");
-        hip_check(hipMalloc((void**)&dev, ncap), "hipMalloc(param ring)");
-        cap = ncap;
-        head = 0;
-    }
     if (head + need > cap) {
-        hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
-        head = 0;
+        if (holds == 0 && need <= cap) {
+            // every region handed out so far is dead once the stream is idle
+            hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+            release_retired();
+            head = 0;
+        } else {
+            // regions handed out earlier may still be read by kernels not yet
+            // launched: keep their buffer and continue in a larger one
+            if (holds == 0) {
+                if (stream) hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+                release_retired();
+                if (host) hip_check(hipHostFree(host), "hipHostFree");
+                if (dev) hip_check(hipFree(dev), "hipFree");
+            } else if (host) {
+                retired.emplace_back(host, dev);
+            }
+            host = nullptr;
+            dev = nullptr;
+            size_t ncap = std::max<size_t>({need, 2 * cap, (size_t)1 << 20});
+            hip_check(hipHostMalloc((void**)&host, ncap, hipHostMallocDefault), "hipHostMalloc(param ring)");
+            hip_check(hipMalloc((void**)&dev, ncap), "hipMalloc(param ring)");
+            cap = ncap;
+            head = 0;
+        }
     }
     std::memcpy(host + head, data, bytes);
     hip_check(hipMemcpyAsync(dev + head, host + head, bytes, hipMemcpyHostToDevice, stream), "hipMemcpyAsync(params)");
@@ -653,6 +676,9 @@ TileOp translate_diag4(const HOp& op, const int* pos, uint64_t tileMask) {
 
 template <typename R>
 void execute_impl(State& s, const std::vector<HOp>& ops, const std::vector<Stage>& stages) {
+    // the pass items pushed below are read by tile passes launched after the
+    // standalone ops in between, which push parameters of their own
+    ParamRing::Hold hold(s.ring);
     constexpr unsigned W = (unsigned)sweep_bits<R>();
     constexpr unsigned FULL = (unsigned)tile_bits_max<R>();
     std::vector<TileOp> items, micro;
@@ -756,7 +782,6 @@ void execute_impl(State& s, const std::vector<HOp>& ops, const std::vector<Stage
             for (int oi : gr.ops) micro.push_back(translate_micro<R>(ops[oi], pos, sl, mats));
             items.push_back(it);
         }
-        if (!stageMfma) fuse_diag_runs(items, first[si], false);
         for (int oi : tail) items.push_back(translate_diag4(ops[oi], pos, g.tileMask));
         flags[si] |= (uint32_t)tail.size() << 8;
         count[si] = items.size() - first[si];
@@ -826,49 +851,6 @@ void execute_impl(State& s, const std::vector<HOp>& ops, const std::vector<Stage
 }
 
 }  // namespace
-
-// Runs of consecutive diagonal items (>= 2) get an OP_DIAG_RUN header so the
-// kernel applies them in one sweep (limits DIAG_RUN_MAX items, DIAG_RUN_GRADS
-// gradient items per run).  Gradient items join a run only in reverse passes,
-// where phi and lambda receive the same diagonal unitaries.
-void fuse_diag_runs(std::vector<TileOp>& items, size_t first, bool rev) {
-    // Off unless FSA_DIAG_RUNS=1: the one-sweep form did not speed up the
-    // measured workloads and is kept for CPU-side validation only.
-    if (!env_flag("FSA_DIAG_RUNS", false)) return;
-    auto diag = [&](const TileOp& t) {
-        return t.kind == OP_PHASE || (t.kind == OP_PAULI && t.lmask == 0) ||
-               (rev && t.kind == OP_GRAD && t.lmask == 0);
-    };
-    std::vector<TileOp> out(items.begin(), items.begin() + (std::ptrdiff_t)first);
-    size_t i = first;
-    while (i < items.size()) {
-        if (!diag(items[i])) { out.push_back(items[i++]); continue; }
-        size_t j = i;
-        uint32_t ngr = 0;
-        TileOp head;
-        std::memset(&head, 0, sizeof(head));
-        head.kind = OP_DIAG_RUN;
-        head.param = NO_PARAM;
-        while (j < items.size() && diag(items[j]) && j - i < DIAG_RUN_MAX) {
-            if (items[j].kind == OP_GRAD) {
-                if (ngr == DIAG_RUN_GRADS) break;
-                head.p[ngr++] = (uint32_t)(j - i);
-            }
-            j++;
-        }
-        if (j - i >= 2) {
-            head.k = (uint32_t)(j - i);
-            head.lmask2 = ngr;
-            out.push_back(head);
-            out.insert(out.end(), items.begin() + (std::ptrdiff_t)i, items.begin() + (std::ptrdiff_t)j);
-        } else {
-            out.push_back(items[i]);
-            j = i + 1;
-        }
-        i = j;
-    }
-    items.swap(out);
-}
 
 template <typename R>
 TileOp translate_item(const HOp& op, const int* pos, uint64_t tileMask, std::vector<cx<R>>& mats) {

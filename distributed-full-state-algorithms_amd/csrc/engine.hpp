@@ -74,14 +74,31 @@ struct Plan {
 std::vector<HOp> relabel_swaps(const std::vector<HOp>& ops, unsigned nq);
 Plan make_plan(const std::vector<HOp>& ops, unsigned nq, int precision);
 
-// pinned-host / device ring for per-launch parameters (matrices, tile ops)
+// Pinned-host / device ring for per-launch parameters (matrices, tile ops).
+// A region handed out by push stays valid until the next push that finds the
+// ring full while no Hold is active: that push waits for the stream and starts
+// over at offset 0.  While a Hold is active (a caller that pushes several
+// regions before launching all their kernels), a full ring moves on to a new,
+// larger buffer and keeps the old one alive until the next such restart.
 struct ParamRing {
     hipStream_t stream = nullptr;
     char* host = nullptr;
     char* dev = nullptr;
     size_t cap = 0, head = 0;
+    int holds = 0;
+    std::vector<std::pair<char*, char*>> retired;   // (host, device) buffers still referenced
     ~ParamRing();
     void* push(const void* data, size_t bytes);   // returns device address, copy queued
+    struct Hold {
+        ParamRing& r;
+        explicit Hold(ParamRing& ring) : r(ring) { r.holds++; }
+        ~Hold() { r.holds--; }
+        Hold(const Hold&) = delete;
+        Hold& operator=(const Hold&) = delete;
+    };
+
+private:
+    void release_retired();
 };
 
 struct State {
@@ -123,37 +140,6 @@ void flush(State& s);
 // position of qubit q or -1; dense matrices are appended to mats.
 template <typename R>
 TileOp translate_item(const HOp& op, const int* pos, uint64_t tileMask, std::vector<cx<R>>& mats);
-
-// Reverse sweep of the adjoint method for an n-qubit state: for k =
-// last..first a gradient item (gadgets with a parameter) then U_k^dag, cut
-// into tile passes over phi and lambda together (or op by op where an op fits
-// no tile).  Pure host code.
-constexpr unsigned REV_MAX_GRADS = 512;     // gradient items per pass (LDS accumulators)
-template <typename R>
-struct RevPlan {
-    struct Pass {
-        bool tiled;
-        BatchStage bs;
-        size_t slot0;                       // first entry of slots for this pass
-        unsigned ngrad;
-        std::vector<int> ops;               // untiled: indices into rev
-    };
-    std::vector<HOp> rev;
-    std::vector<Pass> passes;
-    std::vector<TileOp> items;
-    std::vector<cx<R>> mats;
-    std::vector<uint32_t> slots;            // result slot (1 + parameter) of each gradient item
-    unsigned maxGrad = 0;
-    unsigned m = 0;                         // tile bits
-};
-template <typename R>
-RevPlan<R> build_reverse_plan(const std::vector<HOp>& ops, unsigned n);
-template <typename R>
-int rev_tile_bits();
-
-// give runs of consecutive diagonal items in items[first..] an OP_DIAG_RUN
-// header (rev: gradient items of diagonal strings may join, adjoint passes)
-void fuse_diag_runs(std::vector<TileOp>& items, size_t first, bool rev);
 
 // brackets launches on `stream` with HIP events while profiling is enabled
 class ProfScope {

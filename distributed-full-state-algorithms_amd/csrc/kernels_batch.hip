@@ -108,26 +108,27 @@ __device__ __forceinline__ void grad_apply(cx<R>* phi, cx<R>* lam, uint32_t T, u
         ai += sg * ((double)x.x * (double)y.y - (double)x.y * (double)y.x);
     };
     if (o.kind == OP_PHASE) {
-        const cx<R> f0 = ldc<R>(o.f, 0), f1 = ldc<R>(o.f, 1);
+        const cx<double> f0 = ldc<double>(o.f, 0), f1 = ldc<double>(o.f, 1);
         const uint32_t bq = par64(base & o.gmask);
         for (uint32_t l = threadIdx.x; l < T; l += TB) {
             const cx<R> x = lam[l], y = phi[l];
             term(x, y, l);
-            const cx<R> f = (bq ^ par32(l & o.lmask)) ? f1 : f0;
-            phi[l] = cmul(y, f);
-            lam[l] = cmul(x, f);
+            const cx<double> f = (bq ^ par32(l & o.lmask)) ? f1 : f0;
+            phi[l] = dn<R>(cmul(up<double>(y), f));
+            lam[l] = dn<R>(cmul(up<double>(x), f));
         }
     } else {
-        const cx<R> th = ldc<R>(o.f, 0), fe = ldc<R>(o.f, 1);
-        const cx<R> fo = mk<R>(-fe.x, -fe.y);
+        const cx<double> th = ldc<double>(o.f, 0), fe = ldc<double>(o.f, 1);
+        const cx<double> fo = mk<double>(-fe.x, -fe.y);
         const uint32_t bq = par64(base & o.gmask);
         if (lxy == 0) {
             for (uint32_t l = threadIdx.x; l < T; l += TB) {
                 const cx<R> x = lam[l], y = phi[l];
                 term(x, y, l);
-                const cx<R> f = (bq ^ par32(l & o.lmask2)) ? fo : fe;
-                phi[l] = cfma(cmul(th, y), f, y);
-                lam[l] = cfma(cmul(th, x), f, x);
+                const cx<double> f = (bq ^ par32(l & o.lmask2)) ? fo : fe;
+                const cx<double> yd = up<double>(y), xd = up<double>(x);
+                phi[l] = dn<R>(cfma(cmul(th, yd), f, yd));
+                lam[l] = dn<R>(cfma(cmul(th, xd), f, xd));
             }
         } else {
             const unsigned hb = 31u - (unsigned)__builtin_clz(lxy);
@@ -136,12 +137,13 @@ __device__ __forceinline__ void grad_apply(cx<R>* phi, cx<R>* lam, uint32_t T, u
                 const cx<R> p0 = phi[i0], p1 = phi[i1], l0 = lam[i0], l1 = lam[i1];
                 term(l0, p1, i1);
                 term(l1, p0, i0);
-                const cx<R> f0 = (bq ^ par32(i0 & o.lmask2)) ? fo : fe;
-                const cx<R> f1 = (bq ^ par32(i1 & o.lmask2)) ? fo : fe;
-                phi[i0] = cfma(cmul(th, p0), f1, p1);
-                phi[i1] = cfma(cmul(th, p1), f0, p0);
-                lam[i0] = cfma(cmul(th, l0), f1, l1);
-                lam[i1] = cfma(cmul(th, l1), f0, l0);
+                const cx<double> f0 = (bq ^ par32(i0 & o.lmask2)) ? fo : fe;
+                const cx<double> f1 = (bq ^ par32(i1 & o.lmask2)) ? fo : fe;
+                const cx<double> q0 = up<double>(p0), q1 = up<double>(p1), m0 = up<double>(l0), m1 = up<double>(l1);
+                phi[i0] = dn<R>(cfma(cmul(th, q0), f1, q1));
+                phi[i1] = dn<R>(cfma(cmul(th, q1), f0, q0));
+                lam[i0] = dn<R>(cfma(cmul(th, m0), f1, m1));
+                lam[i1] = dn<R>(cfma(cmul(th, m1), f0, m0));
             }
         }
     }
@@ -154,7 +156,7 @@ __device__ __forceinline__ void grad_apply(cx<R>* phi, cx<R>* lam, uint32_t T, u
 template <typename R, int TB>
 __device__ __forceinline__ void apply_terms(const cx<R>* phiAll, cx<R>* lamOut, uint64_t j0, uint32_t T,
                                             const PauliTermDev* __restrict__ terms, uint32_t nterm,
-                                            double& er, double& ei) {
+                                            double& er, double& ei, double& nn) {
     for (uint32_t l = threadIdx.x; l < T; l += TB) {
         const uint64_t j = j0 + l;
         double ar = 0, ai = 0;
@@ -168,6 +170,7 @@ __device__ __forceinline__ void apply_terms(const cx<R>* phiAll, cx<R>* lamOut, 
             ai += fr * (double)v.y + fi * (double)v.x;
         }
         const cx<R> p = phiAll[j];
+        nn += (double)p.x * (double)p.x + (double)p.y * (double)p.y;
         er += (double)p.x * ar + (double)p.y * ai;
         ei += (double)p.x * ai - (double)p.y * ar;
         lamOut[l] = mk<R>((R)ar, (R)ai);
@@ -179,76 +182,15 @@ __device__ __forceinline__ void apply_terms(const cx<R>* phiAll, cx<R>* lamOut, 
 // workgroup processes, and are written out once per stage (flush_grads).
 constexpr uint32_t MAX_STAGE_GRADS = 512;
 
-// workgroup sums of the first ngr of NG value pairs; thread g < ngr gets sum g
-// in gr[0], gi[0].  red holds 2 * NG * (TB / 64) doubles.
-template <int TB, int NG>
-__device__ __forceinline__ void bsum_multi(double* gr, double* gi, uint32_t ngr, double* red) {
-    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-#pragma unroll
-    for (int g = 0; g < NG; g++) {
-        gr[g] = bwave_sum(gr[g]);
-        gi[g] = bwave_sum(gi[g]);
-    }
-    __syncthreads();                       // red may still be read by a previous sum
-    if (lane == 0) {
-#pragma unroll
-        for (int g = 0; g < NG; g++) {
-            red[2 * (wave * NG + g)] = gr[g];
-            red[2 * (wave * NG + g) + 1] = gi[g];
-        }
-    }
-    __syncthreads();
-    if (threadIdx.x < ngr) {
-        double a = 0, b = 0;
-        for (int w = 0; w < TB / 64; w++) {
-            a += red[2 * (w * NG + threadIdx.x)];
-            b += red[2 * (w * NG + threadIdx.x) + 1];
-        }
-        gr[0] = a;
-        gi[0] = b;
-    }
-}
-
 // the items of one stage on the LDS tile(s); OP_GRAD items add their tile sum
-// into gacc (tiles in a fixed order); OP_DIAG_RUN runs take one sweep
-// RUNS: the plan may hold OP_DIAG_RUN headers (the batch kernel's plans never
-// do, so its instantiation leaves that code out)
-template <typename R, int TB, bool REV, bool RUNS>
+// into gacc (tiles in a fixed order)
+template <typename R, int TB, bool REV>
 __device__ __forceinline__ void run_items(cx<R>* phi, cx<R>* lam, uint32_t T, uint64_t base,
                                           const TileOp* __restrict__ items, uint32_t i0, uint32_t ni,
-                                          const cx<R>* __restrict__ mats, const double* __restrict__ params,
-                                          double* gacc, double* red, DiagEnt* dtab, bool firstTile) {
+                                          const cx<double>* __restrict__ mats, const double* __restrict__ params,
+                                          double* gacc, double* red, bool firstTile) {
     for (uint32_t i = i0; i < i0 + ni; i++) {
         TileOp op = items[i];
-        if (RUNS && op.kind == OP_DIAG_RUN) {
-            const uint32_t cnt = op.k, ngr = REV ? op.lmask2 : 0u;
-            if (threadIdx.x < cnt) {
-                TileOp o = items[i + 1 + threadIdx.x];
-                bind_param(o, params);
-                dtab[threadIdx.x] = diag_entry(o, base);
-            }
-            __syncthreads();
-            if (REV && ngr > 0) {
-                double gr[DIAG_RUN_GRADS], gi[DIAG_RUN_GRADS];
-#pragma unroll
-                for (int g = 0; g < (int)DIAG_RUN_GRADS; g++) { gr[g] = 0; gi[g] = 0; }
-                diag_sweep<R, TB, true, DIAG_RUN_GRADS>(phi, lam, T, dtab, cnt, op.p, ngr, gr, gi);
-                bsum_multi<TB, DIAG_RUN_GRADS>(gr, gi, ngr, red);
-                if (threadIdx.x < ngr) {
-                    const DiagEnt& e = dtab[op.p[threadIdx.x]];
-                    const double vr = e.er * gr[0] - e.ei * gi[0];
-                    const double vi = e.er * gi[0] + e.ei * gr[0];
-                    double* r = gacc + 2 * items[i + 1 + op.p[threadIdx.x]].p[0];
-                    if (firstTile) { r[0] = vr; r[1] = vi; }
-                    else { r[0] += vr; r[1] += vi; }
-                }
-            } else {
-                diag_sweep<R, TB, REV, 0>(phi, lam, T, dtab, cnt, nullptr, 0, nullptr, nullptr);
-            }
-            __syncthreads();
-            i += cnt;
-            continue;
-        }
         if (REV && op.kind == OP_GRAD) {
             double vr, vi;
             const bool fused = op.k == 1u;  // the next item is this gadget's inverse
@@ -270,8 +212,8 @@ __device__ __forceinline__ void run_items(cx<R>* phi, cx<R>* lam, uint32_t T, ui
             continue;
         }
         bind_param(op, params);
-        tile_apply<R, TB, 1>(phi, T, base, op, mats);
-        if (REV) tile_apply<R, TB, 1>(lam, T, base, op, mats);
+        tile_apply<R, TB, 1, 1, double>(phi, T, base, op, mats);
+        if (REV) tile_apply<R, TB, 1, 1, double>(lam, T, base, op, mats);
         __syncthreads();
     }
 }
@@ -295,7 +237,7 @@ __device__ __forceinline__ void flush_grads(const BatchStage& st, const TileOp* 
 template <typename R, int TB, int M>
 __global__ __launch_bounds__(TB) void k_batch(const BatchJob* __restrict__ jobs, unsigned numJobs,
                                               const BatchStage* __restrict__ stages,
-                                              const TileOp* __restrict__ items, const cx<R>* __restrict__ mats,
+                                              const TileOp* __restrict__ items, const cx<double>* __restrict__ mats,
                                               const PauliTermDev* __restrict__ terms,
                                               const double* __restrict__ params, cx<R>* __restrict__ arena,
                                               double* __restrict__ res, unsigned* __restrict__ counter) {
@@ -303,9 +245,8 @@ __global__ __launch_bounds__(TB) void k_batch(const BatchJob* __restrict__ jobs,
     constexpr uint32_t KMAX = TMAX / TB > 0 ? TMAX / TB : 1;
     __shared__ cx<R> lds[2 * TMAX];
     __shared__ uint64_t offHi[KMAX];
-    __shared__ double red[2 * DIAG_RUN_GRADS * (TB / 64)];
+    __shared__ double red[2 * (TB / 64)];
     __shared__ double gacc[2 * MAX_STAGE_GRADS];
-    __shared__ DiagEnt dtab[DIAG_RUN_MAX];
     __shared__ unsigned jobIdx;
     const uint32_t tid = threadIdx.x;
 
@@ -325,16 +266,21 @@ __global__ __launch_bounds__(TB) void k_batch(const BatchJob* __restrict__ jobs,
             for (uint32_t l = tid; l < T; l += TB) phi[l] = mk<R>(l == 0 ? (R)1 : (R)0, (R)0);
             __syncthreads();
             for (uint32_t s = J.fwd0; s < J.fwd0 + J.nfwd; s++)
-                run_items<R, TB, false, false>(phi, lam, T, 0, items, stages[s].item0, stages[s].nitems, mats, params, gacc,
-                                        red, dtab, true);
-            double er = 0, ei = 0;
-            apply_terms<R, TB>(phi, lam, 0, T, terms + J.term0, J.nterm, er, ei);
+                run_items<R, TB, false>(phi, lam, T, 0, items, stages[s].item0, stages[s].nitems, mats, params, gacc,
+                                        red, true);
+            double er = 0, ei = 0, nn = 0, nz = 0;
+            apply_terms<R, TB>(phi, lam, 0, T, terms + J.term0, J.nterm, er, ei, nn);
             bsum2<TB>(er, ei, red);
-            if (tid == 0) { res[2 * (size_t)J.out] = er; res[2 * (size_t)J.out + 1] = ei; }
+            bsum2<TB>(nn, nz, red);
+            if (tid == 0) {
+                res[2 * (size_t)J.out] = er;
+                res[2 * (size_t)J.out + 1] = ei;
+                res[J.nrm] = nn;
+            }
             __syncthreads();
             for (uint32_t s = J.rev0; s < J.rev0 + J.nrev; s++) {
                 const BatchStage st = stages[s];
-                run_items<R, TB, true, false>(phi, lam, T, 0, items, st.item0, st.nitems, mats, params, gacc, red, dtab, true);
+                run_items<R, TB, true>(phi, lam, T, 0, items, st.item0, st.nitems, mats, params, gacc, red, true);
                 flush_grads<TB>(st, items, gacc, res);
             }
             continue;
@@ -396,8 +342,8 @@ __global__ __launch_bounds__(TB) void k_batch(const BatchJob* __restrict__ jobs,
             if (1 < numTiles) load(1);
             for (uint64_t t = 0; t < numTiles; t++) {
                 const uint64_t base = pdep64(t, outMask);
-                run_items<R, TB, REV, false>(lds, lds + TMAX, T, base, items, st.item0, st.nitems, mats, params, gacc,
-                                             red, dtab, t == 0);
+                run_items<R, TB, REV>(lds, lds + TMAX, T, base, items, st.item0, st.nitems, mats, params, gacc,
+                                      red, t == 0);
                 __syncthreads();   // the last item may be a gradient sum still reading other rows
                 // Each thread writes its own rows of tile t back and fills them
                 // with tile t + 1; the wait makes tile t + 1's loads land before the
@@ -438,84 +384,22 @@ __global__ __launch_bounds__(TB) void k_batch(const BatchJob* __restrict__ jobs,
             if (dir == 0) {
                 // the forward pass is complete in the arena (this workgroup's own
                 // stores, ordered by the barrier above): lambda = H phi, <H>
-                double er = 0, ei = 0;
+                double er = 0, ei = 0, nn = 0, nz = 0;
                 const uint64_t num = 1ULL << J.n;
                 for (uint64_t j0 = 0; j0 < num; j0 += TMAX) {
                     const uint32_t T = (uint32_t)((num - j0) < TMAX ? (num - j0) : TMAX);
-                    apply_terms<R, TB>(gphi, glam + j0, j0, T, terms + J.term0, J.nterm, er, ei);
+                    apply_terms<R, TB>(gphi, glam + j0, j0, T, terms + J.term0, J.nterm, er, ei, nn);
                 }
                 bsum2<TB>(er, ei, red);
-                if (tid == 0) { res[2 * (size_t)J.out] = er; res[2 * (size_t)J.out + 1] = ei; }
+                bsum2<TB>(nn, nz, red);
+                if (tid == 0) {
+                    res[2 * (size_t)J.out] = er;
+                    res[2 * (size_t)J.out + 1] = ei;
+                    res[J.nrm] = nn;
+                }
                 __syncthreads();
             }
         }
-    }
-}
-
-// One reverse-sweep pass of the single-state adjoint: every workgroup streams
-// its share of the tiles (fixed assignment, fixed order) of phi and lambda
-// through LDS, applies the stage's inverse gates to both, sums each OP_GRAD
-// item over its tiles in LDS and writes the sums to its own row of `part`
-// (item field p[0] = the item's index within the stage).  k_grad_final then sums the rows in
-// workgroup order, so the result does not depend on timing.
-template <typename R, int TB, int M>
-__global__ __launch_bounds__(TB) void k_rev_pass(cx<R>* __restrict__ phi, cx<R>* __restrict__ lam, unsigned n,
-                                                 BatchStage st, const TileOp* __restrict__ items,
-                                                 const cx<R>* __restrict__ mats, double* __restrict__ part,
-                                                 unsigned ngrad) {
-    constexpr uint32_t TMAX = 1u << M;
-    constexpr uint32_t KMAX = TMAX / TB > 0 ? TMAX / TB : 1;
-    __shared__ cx<R> lds[2 * TMAX];
-    __shared__ uint64_t offHi[KMAX];
-    __shared__ double red[2 * DIAG_RUN_GRADS * (TB / 64)];
-    __shared__ double gacc[2 * MAX_STAGE_GRADS];
-    __shared__ DiagEnt dtab[DIAG_RUN_MAX];
-    const uint32_t tid = threadIdx.x;
-    const uint32_t T = 1u << st.m;
-    const uint32_t K = T > TB ? T / TB : 1u;
-    const uint64_t all = (1ULL << n) - 1ULL;
-    const uint64_t outMask = all & ~st.tileMask;
-    const uint64_t numTiles = 1ULL << (n - st.m);
-    if (tid < K) offHi[tid] = pdep64((uint64_t)tid * TB, st.tileMask);
-    const uint64_t offLo = pdep64(tid, st.tileMask);
-    __syncthreads();
-    double* mine = part + (size_t)blockIdx.x * 2 * ngrad;
-    bool first = true;
-    for (uint64_t t = blockIdx.x; t < numTiles; t += gridDim.x) {
-        const uint64_t base = pdep64(t, outMask);
-        for (uint32_t k = 0; k < K; k++) {
-            const uint32_t l = tid + k * TB;
-            if (l >= T) break;
-            const uint64_t g = base | offLo | offHi[k];
-            lds[l] = phi[g];
-            lds[TMAX + l] = lam[g];
-        }
-        __syncthreads();
-        run_items<R, TB, true, true>(lds, lds + TMAX, T, base, items, 0, st.nitems, mats, nullptr, gacc, red, dtab, first);
-        for (uint32_t k = 0; k < K; k++) {
-            const uint32_t l = tid + k * TB;
-            if (l >= T) break;
-            const uint64_t g = base | offLo | offHi[k];
-            phi[g] = lds[l];
-            lam[g] = lds[TMAX + l];
-        }
-        __syncthreads();
-        first = false;
-    }
-    for (uint32_t i = tid; i < 2 * ngrad; i += TB) mine[i] = gacc[i];
-}
-
-// res[2 slot[i]], res[2 slot[i] + 1] = sum over the rows of part of item i's sums
-__global__ void k_grad_final(const double* __restrict__ part, unsigned rows, unsigned ngrad,
-                             const uint32_t* __restrict__ slots, double* __restrict__ res) {
-    for (unsigned i = threadIdx.x; i < ngrad; i += blockDim.x) {
-        double a = 0, b = 0;
-        for (unsigned w = 0; w < rows; w++) {
-            a += part[(size_t)w * 2 * ngrad + 2 * i];
-            b += part[(size_t)w * 2 * ngrad + 2 * i + 1];
-        }
-        res[2 * (size_t)slots[i]] = a;
-        res[2 * (size_t)slots[i] + 1] = b;
     }
 }
 
@@ -547,7 +431,7 @@ int batch_occupancy(int tb, int m) {
 
 template <typename R>
 bool launch_batch(hipStream_t s, int tb, int m, unsigned grid, const BatchJob* jobs, unsigned numJobs,
-                  const BatchStage* stages, const TileOp* items, const cx<R>* mats, const PauliTermDev* terms,
+                  const BatchStage* stages, const TileOp* items, const cx<double>* mats, const PauliTermDev* terms,
                   const double* params, cx<R>* arena, double* res, unsigned* counter) {
     hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, s, counter);
 #define FSA_BL(TBV, MV)                                                                                    \
@@ -562,40 +446,6 @@ bool launch_batch(hipStream_t s, int tb, int m, unsigned grid, const BatchJob* j
     return false;
 }
 
-template <typename R>
-int rev_tile_bits() { return sizeof(R) == 8 ? 11 : 12; }
-
-// reverse-pass workgroups: one per tile up to what the device holds at once
-template <typename R>
-unsigned rev_grid(uint64_t numTiles) {
-    constexpr int M = sizeof(R) == 8 ? 11 : 12;
-    int dev = 0, cus = 0, blocks = 0;
-    if (hipGetDevice(&dev) != hipSuccess) return 0;
-    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, reinterpret_cast<const void*>(&k_rev_pass<R, 512, M>),
-                                                     512, 0) != hipSuccess || blocks < 1)
-        return 0;
-    return (unsigned)std::min<uint64_t>(numTiles, (uint64_t)cus * blocks);
-}
-
-template <typename R>
-void launch_rev_pass(hipStream_t s, unsigned grid, cx<R>* phi, cx<R>* lam, unsigned n, const BatchStage& st,
-                     const TileOp* items, const cx<R>* mats, double* part, unsigned ngrad, const uint32_t* slots,
-                     double* res) {
-    constexpr int M = sizeof(R) == 8 ? 11 : 12;
-    hipLaunchKernelGGL((k_rev_pass<R, 512, M>), dim3(grid), dim3(512), 0, s, phi, lam, n, st, items, mats, part,
-                       ngrad);
-    if (ngrad) hipLaunchKernelGGL(k_grad_final, dim3(1), dim3(256), 0, s, part, grid, ngrad, slots, res);
-}
-
-template int rev_tile_bits<double>();
-template int rev_tile_bits<float>();
-template unsigned rev_grid<double>(uint64_t);
-template unsigned rev_grid<float>(uint64_t);
-template void launch_rev_pass<double>(hipStream_t, unsigned, cx<double>*, cx<double>*, unsigned, const BatchStage&,
-                                      const TileOp*, const cx<double>*, double*, unsigned, const uint32_t*, double*);
-template void launch_rev_pass<float>(hipStream_t, unsigned, cx<float>*, cx<float>*, unsigned, const BatchStage&,
-                                     const TileOp*, const cx<float>*, double*, unsigned, const uint32_t*, double*);
 template int batch_max_bits<double>();
 template int batch_max_bits<float>();
 template int batch_occupancy<double>(int, int);
@@ -604,7 +454,7 @@ template bool launch_batch<double>(hipStream_t, int, int, unsigned, const BatchJ
                                    const TileOp*, const cx<double>*, const PauliTermDev*, const double*,
                                    cx<double>*, double*, unsigned*);
 template bool launch_batch<float>(hipStream_t, int, int, unsigned, const BatchJob*, unsigned, const BatchStage*,
-                                  const TileOp*, const cx<float>*, const PauliTermDev*, const double*, cx<float>*,
+                                  const TileOp*, const cx<double>*, const PauliTermDev*, const double*, cx<float>*,
                                   double*, unsigned*);
 
 }  // namespace fsa

@@ -376,18 +376,6 @@ __global__ __launch_bounds__(TB, (sizeof(R) == 8 && ROWS >= 16) ? 2 : 4) void k_
         const uint64_t base = ins0_mask(tile, geom.tileMask);
         for (unsigned it = 0; it < nBody; it++) {
             const TileOp& op = items[it];
-            if constexpr (VAR < 2) {   // the host builds no diagonal runs in MFMA passes
-                if (op.kind == OP_DIAG_RUN) {
-                    __shared__ DiagEnt dtab[DIAG_RUN_MAX];
-                    const uint32_t cnt = op.k;
-                    if (tid < cnt) dtab[tid] = diag_entry(items[it + 1 + tid], base);
-                    __syncthreads();
-                    diag_sweep<R, TB, false, 0>(s, nullptr, T, dtab, cnt, nullptr, 0, nullptr, nullptr);
-                    __syncthreads();
-                    it += cnt;
-                    continue;
-                }
-            }
             if constexpr (SW) {
                 if (op.kind == OP_SWEEP) {
                     sweep_lds<R, TB, W>(s, T, op, micro, base, mats);

@@ -23,6 +23,15 @@ __device__ __forceinline__ uint32_t chan_bit(uint32_t l, uint64_t base, uint32_t
 template <typename R>
 __device__ __forceinline__ cx<R> ldc(const double* f, int i) { return mk<R>((R)f[2 * i], (R)f[2 * i + 1]); }
 
+// Amplitudes are stored in precision R and combined in precision C >= R: an
+// FP32 tile worked in FP64 is rounded once per op (on its store), and its gate
+// factors keep FP64 values (FP32-rounded matrices and angles would perturb
+// every amplitude alike, which is the larger error of the two).
+template <typename C, typename R>
+__device__ __forceinline__ cx<C> up(cx<R> v) { return mk<C>((C)v.x, (C)v.y); }
+template <typename R, typename C>
+__device__ __forceinline__ cx<R> dn(cx<C> v) { return mk<R>((R)v.x, (R)v.y); }
+
 // s_waitcnt operand for vmcnt(0) with expcnt and lgkmcnt left at their maxima
 // (gfx9 encoding: vmcnt bits 3:0 and 15:14, expcnt 6:4, lgkmcnt 11:8)
 constexpr int WAIT_VMCNT0 = 0x0F70;
@@ -45,8 +54,8 @@ __device__ __forceinline__ uint32_t mat_offset(uint32_t l, uint32_t b0, uint32_t
     return o;
 }
 
-template <typename R, int K, int TB>
-__device__ __forceinline__ void tile_matk(cx<R>* s, uint32_t T, const TileOp& op, const cx<R>* __restrict__ mat) {
+template <typename R, int K, int TB, typename C = R>
+__device__ __forceinline__ void tile_matk(cx<R>* s, uint32_t T, const TileOp& op, const cx<C>* __restrict__ mat) {
     constexpr int D = 1 << K;
     const uint32_t b0 = 1u << op.p[0];
     const uint32_t b1 = K > 1 ? 1u << op.p[1] : 0u;
@@ -57,25 +66,25 @@ __device__ __forceinline__ void tile_matk(cx<R>* s, uint32_t T, const TileOp& op
 #define FSA_OFF(l) mat_offset<K>((l), b0, b1, b2, b3, b4)
     const uint32_t groups = T >> K;
     const uint32_t tid = threadIdx.x;
-    constexpr int OUTMAX = sizeof(R) == 8 ? 8 : 16;
+    constexpr int OUTMAX = sizeof(C) == 8 ? 8 : 16;
     constexpr int OUT = D < OUTMAX ? D : OUTMAX;
     // whole groups per thread, written in place; a full tile has >= TB groups only when D <= OUT
     if constexpr (D <= OUT) if (groups >= (uint32_t)TB) {
         for (uint32_t g = tid; g < groups; g += TB) {
-            const cx<R>* m = mat;
+            const cx<C>* m = mat;
             const uint32_t b = ins0_mask32(g, tm);
-            cx<R> acc[D];
+            cx<C> acc[D];
 #pragma unroll
-            for (int o = 0; o < D; o++) acc[o] = mk<R>(0, 0);
+            for (int o = 0; o < D; o++) acc[o] = mk<C>(0, 0);
             // input loop kept rolled: D accumulators and one matrix column live at a time
 #pragma unroll 1
             for (int l = 0; l < D; l++) {
-                cx<R> x = s[b | FSA_OFF((uint32_t)l)];
+                cx<C> x = up<C>(s[b | FSA_OFF((uint32_t)l)]);
 #pragma unroll
                 for (int o = 0; o < D; o++) acc[o] = cfma(acc[o], m[o * D + l], x);
             }
 #pragma unroll
-            for (int o = 0; o < D; o++) s[b | FSA_OFF((uint32_t)o)] = acc[o];
+            for (int o = 0; o < D; o++) s[b | FSA_OFF((uint32_t)o)] = dn<R>(acc[o]);
         }
         return;
     }
@@ -92,18 +101,18 @@ __device__ __forceinline__ void tile_matk(cx<R>* s, uint32_t T, const TileOp& op
         const uint32_t g = g0 + gl;
         const bool active = g < groups;
         const uint32_t b = ins0_mask32(active ? g : 0u, tm);
-        cx<R> acc[OUT];
+        cx<C> acc[OUT];
 #pragma unroll
-        for (int o = 0; o < OUT; o++) acc[o] = mk<R>(0, 0);
+        for (int o = 0; o < OUT; o++) acc[o] = mk<C>(0, 0);
         if (GPR >= 64) {
             // a wave spans one `sub`, so its output rows are wave-uniform: read the
             // matrix through the scalar unit instead of one vector load per lane
             const uint32_t r0u = __builtin_amdgcn_readfirstlane(r0);
-            const cx<R>* mrow = mat + (size_t)r0u * D;
+            const cx<C>* mrow = mat + (size_t)r0u * D;
             if (active) {
 #pragma unroll 1
                 for (int l = 0; l < D; l++) {
-                    cx<R> x = s[b | FSA_OFF((uint32_t)l)];
+                    cx<C> x = up<C>(s[b | FSA_OFF((uint32_t)l)]);
 #pragma unroll
                     for (int o = 0; o < (int)OUTS; o++) acc[o] = cfma(acc[o], mrow[o * D + l], x);
                 }
@@ -111,7 +120,7 @@ __device__ __forceinline__ void tile_matk(cx<R>* s, uint32_t T, const TileOp& op
         } else if (active) {
 #pragma unroll 1
             for (int l = 0; l < D; l++) {
-                cx<R> x = s[b | FSA_OFF((uint32_t)l)];
+                cx<C> x = up<C>(s[b | FSA_OFF((uint32_t)l)]);
 #pragma unroll
                 for (int o = 0; o < (int)OUTS; o++) acc[o] = cfma(acc[o], mat[(r0 + o) * D + l], x);
             }
@@ -119,7 +128,7 @@ __device__ __forceinline__ void tile_matk(cx<R>* s, uint32_t T, const TileOp& op
         __syncthreads();
         if (active) {
 #pragma unroll
-            for (int o = 0; o < (int)OUTS; o++) s[b | FSA_OFF(r0 + o)] = acc[o];
+            for (int o = 0; o < (int)OUTS; o++) s[b | FSA_OFF(r0 + o)] = dn<R>(acc[o]);
         }
     }
 #undef FSA_OFF
@@ -253,9 +262,9 @@ __device__ __forceinline__ void tile_matk_mfma(cx<R>* s, uint32_t T, const TileO
 // the matrices read from HBM.  Each instantiation stays within 128 VGPRs so
 // two 64 KiB workgroups share a CU.
 // lmat: the pass's LDS matrix area (VAR 2: every MFMA item has an offset in it)
-template <typename R, int TB, int VAR, int MAXT = 1>
+template <typename R, int TB, int VAR, int MAXT = 1, typename C = R>
 __device__ __forceinline__ void tile_apply(cx<R>* s, uint32_t T, uint64_t base, const TileOp& op,
-                           const cx<R>* __restrict__ mats, const R* lmat = nullptr) {
+                           const cx<C>* __restrict__ mats, const R* lmat = nullptr) {
     const uint32_t tid = threadIdx.x;
     switch (op.kind) {
     case OP_MAT1: {
@@ -263,12 +272,12 @@ __device__ __forceinline__ void tile_apply(cx<R>* s, uint32_t T, uint64_t base, 
         const uint32_t tb = 1u << op.p[0];
         const uint32_t lc = op.lmask;
         const uint32_t nIts = T >> (1 + __builtin_popcount(lc));
-        const cx<R> g00 = ldc<R>(op.f, 0), g01 = ldc<R>(op.f, 1), g10 = ldc<R>(op.f, 2), g11 = ldc<R>(op.f, 3);
+        const cx<C> g00 = ldc<C>(op.f, 0), g01 = ldc<C>(op.f, 1), g10 = ldc<C>(op.f, 2), g11 = ldc<C>(op.f, 3);
         for (uint32_t j = tid; j < nIts; j += TB) {
             uint32_t i0 = ins0_mask32(j, lc | tb) | lc;
-            cx<R> v0 = s[i0], v1 = s[i0 | tb];
-            s[i0] = cfma(cmul(g00, v0), g01, v1);
-            s[i0 | tb] = cfma(cmul(g10, v0), g11, v1);
+            cx<C> v0 = up<C>(s[i0]), v1 = up<C>(s[i0 | tb]);
+            s[i0] = dn<R>(cfma(cmul(g00, v0), g01, v1));
+            s[i0 | tb] = dn<R>(cfma(cmul(g10, v0), g11, v1));
         }
         return;
     }
@@ -283,41 +292,42 @@ __device__ __forceinline__ void tile_apply(cx<R>* s, uint32_t T, uint64_t base, 
         return;
     }
     case OP_PAULI: {
-        const cx<R> th = ldc<R>(op.f, 0), fe = ldc<R>(op.f, 1);
-        const cx<R> fo = mk<R>(-fe.x, -fe.y);
+        const cx<C> th = ldc<C>(op.f, 0), fe = ldc<C>(op.f, 1);
+        const cx<C> fo = mk<C>(-fe.x, -fe.y);
         const uint32_t bp = par64(base & op.gmask);
         const uint32_t lxy = op.lmask, lyz = op.lmask2;
         if (lxy == 0) {
             for (uint32_t l = tid; l < T; l += TB) {
-                cx<R> v = s[l];
-                cx<R> f = (bp ^ par32(l & lyz)) ? fo : fe;
-                s[l] = cfma(cmul(th, v), f, v);
+                cx<C> v = up<C>(s[l]);
+                cx<C> f = (bp ^ par32(l & lyz)) ? fo : fe;
+                s[l] = dn<R>(cfma(cmul(th, v), f, v));
             }
             return;
         }
         const unsigned hb = 31u - (unsigned)__builtin_clz(lxy);
         for (uint32_t j = tid; j < (T >> 1); j += TB) {
             uint32_t i0 = ins0_32(j, hb), i1 = i0 ^ lxy;
-            cx<R> v0 = s[i0], v1 = s[i1];
-            cx<R> f0 = (bp ^ par32(i0 & lyz)) ? fo : fe;
-            cx<R> f1 = (bp ^ par32(i1 & lyz)) ? fo : fe;
-            s[i0] = cfma(cmul(th, v0), f1, v1);
-            s[i1] = cfma(cmul(th, v1), f0, v0);
+            cx<C> v0 = up<C>(s[i0]), v1 = up<C>(s[i1]);
+            cx<C> f0 = (bp ^ par32(i0 & lyz)) ? fo : fe;
+            cx<C> f1 = (bp ^ par32(i1 & lyz)) ? fo : fe;
+            s[i0] = dn<R>(cfma(cmul(th, v0), f1, v1));
+            s[i1] = dn<R>(cfma(cmul(th, v1), f0, v0));
         }
         return;
     }
     case OP_PHASE: {
-        const cx<R> f0 = ldc<R>(op.f, 0), f1 = ldc<R>(op.f, 1);
+        const cx<C> f0 = ldc<C>(op.f, 0), f1 = ldc<C>(op.f, 1);
         const uint32_t bp = par64(base & op.gmask);
         for (uint32_t l = tid; l < T; l += TB)
-            s[l] = cmul(s[l], (bp ^ par32(l & op.lmask)) ? f1 : f0);
+            s[l] = dn<R>(cmul(up<C>(s[l]), (bp ^ par32(l & op.lmask)) ? f1 : f0));
         return;
     }
     case OP_MATK: {
         if constexpr (VAR == 0) return;
-        const cx<R>* mat = mats + op.mat_off;
+        const cx<C>* mat = mats + op.mat_off;
         if (op.lmask2 == 1u) {        // host chose the MFMA form (full tile, k = 3, 4 or 5)
             if constexpr (VAR >= 2) {
+                static_assert(sizeof(C) == sizeof(R), "MFMA tiles compute in their storage precision");
                 constexpr bool LDSA = VAR == 2;   // VAR 2: every MFMA matrix of the pass is in LDS
                 const R* M = LDSA ? lmat + op.lmask : reinterpret_cast<const R*>(mat);
                 if (op.k == 3) tile_matk_mfma<R, 3, TB, MAXT, LDSA>(s, T, op, M);
@@ -328,166 +338,87 @@ __device__ __forceinline__ void tile_apply(cx<R>* s, uint32_t T, uint64_t base, 
         }
         if constexpr (VAR >= 2) {
             // the MFMA variants also carry two-target gates (VALU, in place)
-            if (op.k == 2) tile_matk<R, 2, TB>(s, T, op, mat);
+            if (op.k == 2) tile_matk<R, 2, TB, C>(s, T, op, mat);
             return;
         }
         switch (op.k) {
-        case 1: tile_matk<R, 1, TB>(s, T, op, mat); break;
-        case 2: tile_matk<R, 2, TB>(s, T, op, mat); break;
-        case 3: tile_matk<R, 3, TB>(s, T, op, mat); break;
-        case 4: tile_matk<R, 4, TB>(s, T, op, mat); break;
-        case 5: tile_matk<R, 5, TB>(s, T, op, mat); break;
+        case 1: tile_matk<R, 1, TB, C>(s, T, op, mat); break;
+        case 2: tile_matk<R, 2, TB, C>(s, T, op, mat); break;
+        case 3: tile_matk<R, 3, TB, C>(s, T, op, mat); break;
+        case 4: tile_matk<R, 4, TB, C>(s, T, op, mat); break;
+        case 5: tile_matk<R, 5, TB, C>(s, T, op, mat); break;
         default: break;
         }
         return;
     }
     case OP_DEPHASE1: {
-        const R fac = (R)op.f[0];
+        const C fac = (C)op.f[0];
         const uint32_t cq = op.p[0], ca = op.p[1];
         for (uint32_t l = tid; l < T; l += TB)
-            if (chan_bit(l, base, cq) ^ chan_bit(l, base, ca)) s[l] = cscale(s[l], fac);
+            if (chan_bit(l, base, cq) ^ chan_bit(l, base, ca)) s[l] = dn<R>(cscale(up<C>(s[l]), fac));
         return;
     }
     case OP_DEPHASE2: {
-        const R fac = (R)op.f[0];
+        const C fac = (C)op.f[0];
         for (uint32_t l = tid; l < T; l += TB) {
             uint32_t f = (chan_bit(l, base, op.p[0]) ^ chan_bit(l, base, op.p[1])) |
                          (chan_bit(l, base, op.p[2]) ^ chan_bit(l, base, op.p[3]));
-            if (f) s[l] = cscale(s[l], fac);
+            if (f) s[l] = dn<R>(cscale(up<C>(s[l]), fac));
         }
         return;
     }
     case OP_DEPOL1: {
-        const R c1 = (R)op.f[0], c2 = (R)op.f[1], c3 = (R)op.f[2];
+        const C c1 = (C)op.f[0], c2 = (C)op.f[1], c3 = (C)op.f[2];
         const uint32_t bq = 1u << op.p[0], ba = 1u << op.p[1];
         for (uint32_t j = tid; j < (T >> 2); j += TB) {
             uint32_t i = ins0_mask32(j, bq | ba);
-            cx<R> v00 = s[i], v01 = s[i | bq], v10 = s[i | ba], v11 = s[i | bq | ba];
-            s[i] = mk<R>(c2 * v00.x + c1 * v11.x, c2 * v00.y + c1 * v11.y);
-            s[i | bq] = cscale(v01, c3);
-            s[i | ba] = cscale(v10, c3);
-            s[i | bq | ba] = mk<R>(c1 * v00.x + c2 * v11.x, c1 * v00.y + c2 * v11.y);
+            cx<C> v00 = up<C>(s[i]), v01 = up<C>(s[i | bq]), v10 = up<C>(s[i | ba]), v11 = up<C>(s[i | bq | ba]);
+            s[i] = dn<R>(mk<C>(c2 * v00.x + c1 * v11.x, c2 * v00.y + c1 * v11.y));
+            s[i | bq] = dn<R>(cscale(v01, c3));
+            s[i | ba] = dn<R>(cscale(v10, c3));
+            s[i | bq | ba] = dn<R>(mk<C>(c1 * v00.x + c2 * v11.x, c1 * v00.y + c2 * v11.y));
         }
         return;
     }
     case OP_DAMP: {
-        const R prob = (R)op.f[0], c1 = (R)op.f[1], c2 = (R)op.f[2];
+        const C prob = (C)op.f[0], c1 = (C)op.f[1], c2 = (C)op.f[2];
         const uint32_t bq = 1u << op.p[0], ba = 1u << op.p[1];
         for (uint32_t j = tid; j < (T >> 2); j += TB) {
             uint32_t i = ins0_mask32(j, bq | ba);
-            cx<R> v00 = s[i], v01 = s[i | bq], v10 = s[i | ba], v11 = s[i | bq | ba];
-            s[i] = mk<R>(v00.x + prob * v11.x, v00.y + prob * v11.y);
-            s[i | bq] = cscale(v01, c1);
-            s[i | ba] = cscale(v10, c1);
-            s[i | bq | ba] = cscale(v11, c2);
+            cx<C> v00 = up<C>(s[i]), v01 = up<C>(s[i | bq]), v10 = up<C>(s[i | ba]), v11 = up<C>(s[i | bq | ba]);
+            s[i] = dn<R>(mk<C>(v00.x + prob * v11.x, v00.y + prob * v11.y));
+            s[i | bq] = dn<R>(cscale(v01, c1));
+            s[i | ba] = dn<R>(cscale(v10, c1));
+            s[i | bq | ba] = dn<R>(cscale(v11, c2));
         }
         return;
     }
     case OP_DEPOL2: {
         if constexpr (VAR == 0) return;
-        const R c1 = (R)op.f[0], c2 = (R)op.f[1], sc = (R)op.f[2];
+        const C c1 = (C)op.f[0], c2 = (C)op.f[1], sc = (C)op.f[2];
         const uint32_t b0 = 1u << op.p[0], b1 = 1u << op.p[1], b2 = 1u << op.p[2], b3 = 1u << op.p[3];
         const uint32_t mk4 = b0 | b1 | b2 | b3;
         for (uint32_t j = tid; j < (T >> 4); j += TB) {
             uint32_t i = ins0_mask32(j, mk4);
             // block-diagonal quartet: row bits equal column bits on both qubits
             const uint32_t d0 = i, d1 = i | b0 | b2, d2 = i | b1 | b3, d3 = i | mk4;
-            cx<R> v0 = s[d0], v1 = s[d1], v2 = s[d2], v3 = s[d3];
-            cx<R> sum = cadd(cadd(v0, v1), cadd(v2, v3));
-            s[d0] = mk<R>(c1 * v0.x + c2 * sum.x, c1 * v0.y + c2 * sum.y);
-            s[d1] = mk<R>(c1 * v1.x + c2 * sum.x, c1 * v1.y + c2 * sum.y);
-            s[d2] = mk<R>(c1 * v2.x + c2 * sum.x, c1 * v2.y + c2 * sum.y);
-            s[d3] = mk<R>(c1 * v3.x + c2 * sum.x, c1 * v3.y + c2 * sum.y);
+            cx<C> v0 = up<C>(s[d0]), v1 = up<C>(s[d1]), v2 = up<C>(s[d2]), v3 = up<C>(s[d3]);
+            cx<C> sum = cadd(cadd(v0, v1), cadd(v2, v3));
+            s[d0] = dn<R>(mk<C>(c1 * v0.x + c2 * sum.x, c1 * v0.y + c2 * sum.y));
+            s[d1] = dn<R>(mk<C>(c1 * v1.x + c2 * sum.x, c1 * v1.y + c2 * sum.y));
+            s[d2] = dn<R>(mk<C>(c1 * v2.x + c2 * sum.x, c1 * v2.y + c2 * sum.y));
+            s[d3] = dn<R>(mk<C>(c1 * v3.x + c2 * sum.x, c1 * v3.y + c2 * sum.y));
 #pragma unroll
             for (int l = 0; l < 16; l++) {
                 if (l == 0 || l == 5 || l == 10 || l == 15) continue;
                 uint32_t x = i | ((l & 1) ? b0 : 0u) | ((l & 2) ? b1 : 0u) | ((l & 4) ? b2 : 0u) | ((l & 8) ? b3 : 0u);
-                s[x] = cscale(s[x], sc);
+                s[x] = dn<R>(cscale(up<C>(s[x]), sc));
             }
         }
         return;
     }
     default:
         return;
-    }
-}
-
-
-// ---- runs of diagonal ops -----------------------------------------------------
-// An OP_DIAG_RUN header (k = number of items that follow it) marks k
-// consecutive items that are all diagonal in the tile: OP_PHASE, OP_PAULI
-// with no X/Y qubit in the tile (lmask == 0) and, in adjoint reverse passes,
-// OP_GRAD of a diagonal Pauli string.  They are applied in ONE sweep over the
-// tile instead of one sweep (and one barrier) each: every amplitude is read
-// once, multiplied by each op's parity-selected factor, and written once.
-// Gradient items of the run (header p[0..lmask2-1] = their positions in the
-// run) are all evaluated on the run's input state: a diagonal P commutes with
-// the run's diagonal unitaries, and those act on phi and lambda alike, so
-// <D lambda|P|D phi> = <lambda|P|phi> for every such D.
-
-struct DiagEnt {
-    double er, ei;        // factor on even parity (gradient items: i^#Y)
-    double orr, oi;       // factor on odd parity
-    uint32_t mask;        // tile-local parity mask
-    uint32_t bp;          // parity of the tile's base index under the out-of-tile mask
-    uint32_t grad;        // 1 for a gradient item (no factor applied)
-    uint32_t pad;
-};
-
-// item -> table entry for the tile with base index `base` (gadget factors already bound)
-__device__ __forceinline__ DiagEnt diag_entry(const TileOp& o, uint64_t base) {
-    DiagEnt e;
-    e.bp = par64(base & o.gmask);
-    e.grad = 0;
-    e.pad = 0;
-    if (o.kind == OP_PHASE) {
-        e.mask = o.lmask;
-        e.er = o.f[0]; e.ei = o.f[1];
-        e.orr = o.f[2]; e.oi = o.f[3];
-    } else if (o.kind == OP_PAULI) {          // th * v + (+-fe) * v
-        e.mask = o.lmask2;
-        e.er = o.f[0] + o.f[2]; e.ei = o.f[1] + o.f[3];
-        e.orr = o.f[0] - o.f[2]; e.oi = o.f[1] - o.f[3];
-    } else {                                  // OP_GRAD
-        e.mask = o.lmask2;
-        e.er = o.f[0]; e.ei = o.f[1];
-        e.orr = 0; e.oi = 0;
-        e.grad = 1;
-    }
-    return e;
-}
-
-// One sweep over the tile(s) with the run's table in LDS.  TWO: apply to s2
-// as well (phi and lambda).  NG > 0: accumulate sum_l sign_g(l) conj(s2_l) s_l
-// for the run's first NG gradient entries (positions gidx) into gr/gi.
-template <typename R, int TB, bool TWO, int NG>
-__device__ __forceinline__ void diag_sweep(cx<R>* s, cx<R>* s2, uint32_t T, const DiagEnt* tab, uint32_t count,
-                                           const uint32_t* gidx, uint32_t ngr, double* gr, double* gi) {
-    for (uint32_t l = threadIdx.x; l < T; l += TB) {
-        cx<R> v = s[l];
-        cx<R> w = TWO ? s2[l] : v;
-        if constexpr (NG > 0) {
-            const double xr = (double)w.x * (double)v.x + (double)w.y * (double)v.y;
-            const double xi = (double)w.x * (double)v.y - (double)w.y * (double)v.x;
-#pragma unroll
-            for (int g = 0; g < NG; g++) {
-                if ((uint32_t)g >= ngr) break;
-                const DiagEnt& e = tab[gidx[g]];
-                const double sg = (e.bp ^ par32(l & e.mask)) ? -1.0 : 1.0;
-                gr[g] += sg * xr;
-                gi[g] += sg * xi;
-            }
-        }
-        for (uint32_t j = 0; j < count; j++) {
-            const DiagEnt& e = tab[j];
-            if (e.grad) continue;
-            const bool odd = (e.bp ^ par32(l & e.mask)) != 0;
-            const cx<R> f = odd ? mk<R>((R)e.orr, (R)e.oi) : mk<R>((R)e.er, (R)e.ei);
-            v = cmul(v, f);
-            if (TWO) w = cmul(w, f);
-        }
-        s[l] = v;
-        if (TWO) s2[l] = w;
     }
 }
 

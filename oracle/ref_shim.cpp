@@ -130,4 +130,47 @@ void ref_dm_partialTrace(const double* amps, unsigned N, const unsigned* t, unsi
     store(o, out);
 }
 
+// ---- timing on a persistent state (CPU-baseline calibration) -------------------
+// The entry points above copy the state in and out around each call; these
+// keep one reference state alive and time only the reference call itself.
+// op codes: 0 oneTargGate, 1 manyCtrlOneTargGate (2 controls), 2 swapGate,
+// 3 phaseGadget; density matrices: 10 oneQubitDephasing, 11
+// twoQubitDephasing, 12 oneQubitDepolarising, 13 damping, 14
+// expecPauliString (one term per call).  q holds the qubits the op takes.
+void* ref_time_state_new(unsigned n, int dm) {
+    ensure_mpi();
+    if (dm) return new DensityMatrix(n);
+    return new StateVector(n);
+}
+void ref_time_state_free(void* s, int dm) {
+    if (dm) delete static_cast<DensityMatrix*>(s);
+    else delete static_cast<StateVector*>(s);
+}
+double ref_time_op(void* sp, int dm, int op, const unsigned* q, const double* g, double x, int reps) {
+    StateVector& s = dm ? *static_cast<DensityMatrix*>(sp) : *static_cast<StateVector*>(sp);
+    const AmpMatrix m = op <= 1 ? to_matrix(g, 2) : getZeroMatrix(2);
+    const double t0 = omp_get_wtime();
+    for (int r = 0; r < reps; r++) {
+        switch (op) {
+        case 0: distributed_statevector_oneTargGate(s, q[0], m); break;
+        case 1: distributed_statevector_manyCtrlOneTargGate(s, NatArray{q[1], q[2]}, q[0], m); break;
+        case 2: distributed_statevector_swapGate(s, q[0], q[1]); break;
+        case 3: distributed_statevector_phaseGadget(s, NatArray(q, q + 4), x); break;
+        case 10: distributed_densitymatrix_oneQubitDephasing(static_cast<DensityMatrix&>(s), q[0], x); break;
+        case 11: distributed_densitymatrix_twoQubitDephasing(static_cast<DensityMatrix&>(s), q[0], q[1], x); break;
+        case 12: distributed_densitymatrix_oneQubitDepolarising(static_cast<DensityMatrix&>(s), q[0], x); break;
+        case 13: distributed_densitymatrix_damping(static_cast<DensityMatrix&>(s), q[0], x); break;
+        case 14: {
+            DensityMatrix& d = static_cast<DensityMatrix&>(s);
+            NatArray codes(d.numQubits, 0);
+            for (unsigned i = 0; i < 4 && i < d.numQubits; i++) codes[q[i]] = 1 + i % 3;
+            (void)distributed_densitymatrix_expecPauliString(d, RealArray{x}, codes);
+            break;
+        }
+        default: return -1.0;
+        }
+    }
+    return (omp_get_wtime() - t0) / reps;
+}
+
 }  // extern "C"

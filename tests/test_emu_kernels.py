@@ -306,3 +306,19 @@ def test_batch_kernel_on_cpu(drive, orc, tmp_path, prec, tol):
     # tile bits 11: the 12..14-qubit jobs stream through LDS, the others stay in it
     out = _run(drive, ["batch", path, 512, 11, 4, prec])
     assert out["expec_abs_err"] <= tol and out["grad_rel_err"] <= tol, out
+
+
+def test_standalone_dense_gate_between_passes_on_cpu(drive, orc, tmp_path):
+    # an 8-target gate fits no tile, so it runs as a standalone kernel between
+    # two tile passes and pushes its 1 MiB matrix into the parameter ring that
+    # still holds the second pass's items (pushed before any launch): the ring
+    # must keep them alive, where it once freed them (use after free)
+    rng = np.random.default_rng(36)
+    n = 12
+    ops = [("oneTargGate", (3, _unitary(rng, 2))), ("oneTargGate", (9, _unitary(rng, 2))),
+           ("manyTargGate", ([5, 0, 7, 2, 11, 3, 9, 6], _unitary(rng, 256))),
+           ("oneTargGate", (0, _unitary(rng, 2))), ("oneTargGate", (7, _unitary(rng, 2)))]
+    path = tmp_path / "ring.bin"
+    _circuit_file(path, orc, n, ops, rng)
+    out = _run(drive, ["circuit", path])
+    assert out["rel_err"] <= 1e-12 and out["passes"] == 3 and out["tiled"] == 2, out

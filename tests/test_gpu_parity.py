@@ -364,3 +364,62 @@ def test_deterministic_reruns(fsa):
         outs.append(psi.download())
         norms.append(psi.norm2())
     assert np.array_equal(outs[0], outs[1]) and norms[0] == norms[1]
+
+
+@pytest.mark.parametrize("n", [12, 16])
+def test_wide_dense_gate_between_fused_passes(fsa, orc, n):
+    """An 8-target gate fits no tile: it runs as a standalone kernel between two
+    tile passes and pushes its 1 MiB matrix while the later pass's items are
+    already in the parameter ring (they must stay valid)."""
+    rng = np.random.default_rng(40 + n)
+    psi = fsa.StateVector(n, mode=1)
+    v = rs(rng, n)
+    psi.upload(v)
+    ref = v.copy()
+    ops = [("oneTargGate", (3, rm(rng, 2))), ("oneTargGate", (9, rm(rng, 2))),
+           ("manyTargGate", ([5, 0, 7, 2, 11, 3, 9, 6], np.linalg.svd(rm(rng, 256))[0])),
+           ("oneTargGate", (0, rm(rng, 2))), ("oneTargGate", (7, rm(rng, 2))),
+           ("manyTargGate", ([1, 10, 4, 8, 2, 0, 11, 5, 3], np.linalg.svd(rm(rng, 512))[0])),
+           ("pauliGadget", ([2, 10], [1, 2], 0.4)), ("oneTargGate", (11, rm(rng, 2)))]
+    for _ in range(3):                      # several flushes reuse and outgrow the ring
+        for name, args in ops:
+            apply_sv(fsa, orc, psi, ref, name, args)
+        psi.flush()
+    err = rel(psi.download(), ref)
+    assert err <= TOL64, err
+
+
+def test_dm_kraus_map_on_four_targets(fsa, orc):
+    # a 4-target Kraus map is an 8-target superoperator: the standalone dense path
+    rng = np.random.default_rng(41)
+    N = 6
+    rho = fsa.DensityMatrix(N, mode=1)
+    v = rs(rng, 2 * N)
+    rho.upload(v)
+    ref = v.copy()
+    kraus = np.stack([rm(rng, 16) * 0.3 for _ in range(3)])
+    apply_dm(fsa, orc, rho, ref, "oneQubitDepolarising", (1, 0.2))
+    apply_dm(fsa, orc, rho, ref, "damping", (4, 0.3))
+    apply_dm(fsa, orc, rho, ref, "krausMap", (kraus, [4, 0, 5, 2]))
+    apply_dm(fsa, orc, rho, ref, "oneQubitDephasing", (0, 0.1))
+    apply_dm(fsa, orc, rho, ref, "manyTargGate", ([3, 5], rm(rng, 4)))
+    assert rel(rho.download(), ref) <= TOL64
+
+
+def test_copy_is_ordered_before_later_source_work(fsa, orc):
+    """copy_from queues the copy on the destination's stream; an op on the source
+    right after it must not overwrite the source before the copy has read it."""
+    rng = np.random.default_rng(42)
+    n = 26
+    src = fsa.StateVector(n, mode=0)
+    src.init_gaussian(77)
+    dst = fsa.StateVector(n, mode=0)
+    dst.copy_from(src)
+    u = np.linalg.svd(rm(rng, 2))[0]
+    for t in range(6):                      # work queued on the source's stream at once
+        fsa.distributed_statevector_oneTargGate(src, t + 20, u)
+    want = orc.init_gaussian(n, 77)
+    assert rel(dst.download(), want) <= 1e-15
+    for t in range(6):
+        orc.sv_oneTargGate(want, t + 20, u)
+    assert rel(src.download(), want) <= TOL64
