@@ -137,6 +137,18 @@ int fsa_densitymatrix_expecPauliString(fsa_state* rho, const double* coeffs, uns
 int fsa_densitymatrix_partialTrace(fsa_state* rho, const unsigned* targets, unsigned numTargets,
                                    fsa_state** out);                              /* :347-406 */
 
+/* No counterpart in the reference: fsa_densitymatrix_expecPauliString for
+ * callers that cannot allocate (code under HIP graph capture, serving loops,
+ * harnesses that forbid hipMalloc).  Same arguments, same kernels, same result
+ * bit for bit; its term list and partial sums live in a static 4 MiB device
+ * array per GPU instead of memory the library allocates, and it calls no
+ * hipMalloc/hipFree.  numTerms * (N + 6) must stay within about 0.5 M
+ * (FSA_ERR_SIZE otherwise); the state must have no queued operations
+ * (FSA_ERR_ARG: flush first, a flush may allocate).  Calls that overlap on one
+ * GPU share that array and race. */
+int fsa_densitymatrix_expecPauliString_noalloc(fsa_state* rho, const double* coeffs, unsigned numTerms,
+                                               const unsigned* allPaulis, double* out /* [2] */);
+
 /* The reference's local twoQubitDepolarising is not trace preserving (its
  * block-diagonal coefficient is 1-4p/5 where the textbook channel needs
  * 1-16p/15).  By default this library applies the textbook channel; passing 1
