@@ -419,7 +419,9 @@ def test_copy_is_ordered_before_later_source_work(fsa, orc):
     for t in range(6):                      # work queued on the source's stream at once
         fsa.distributed_statevector_oneTargGate(src, t + 20, u)
     want = orc.init_gaussian(n, 77)
-    assert rel(dst.download(), want) <= 1e-15
+    # the device and oracle laws agree to 1e-13 (normalisation sums in another
+    # order); a copy of a half-updated source would be off by O(1)
+    assert rel(dst.download(), want) <= 1e-13
     for t in range(6):
         orc.sv_oneTargGate(want, t + 20, u)
     assert rel(src.download(), want) <= TOL64
