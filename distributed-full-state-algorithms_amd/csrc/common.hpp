@@ -115,6 +115,7 @@ enum OpKind : uint32_t {
     OP_SWEEP = 11,   // tile pass item: a run of ops applied to register groups (see kernels_tile.hip)
     OP_GRAD = 12,    // batched adjoint: accumulate <lambda|P|phi> over the tile (see kernels_batch.hip)
     OP_DIAG4 = 14,   // diagonal factor f[(parity1 << 1) | parity2] (pass-end item applied at the write-back)
+    OP_REMAP = 15,   // register-resident passes: move amplitudes to another slot layout (see kernels_rtile.hip)
 };
 
 // One item of a fused tile pass, or one micro-op inside a sweep item.

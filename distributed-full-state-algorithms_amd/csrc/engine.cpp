@@ -686,6 +686,17 @@ void execute_impl(State& s, const std::vector<HOp>& ops, const std::vector<Stage
     std::vector<size_t> first(stages.size(), 0), count(stages.size(), 0);
     std::vector<uint32_t> flags(stages.size(), 0), matLds(stages.size(), 0);
     std::vector<TileGeom> geoms(stages.size());
+    std::vector<RTilePass> rpass(stages.size());
+    // FP64 passes over full tiles may run register-resident (k_rtile) when
+    // every op has a register form.  Measured at 30 qubits, a remap costs about
+    // 1 ms and an exchange between lanes about 0.5 ms, while a dense gate of 3-5
+    // targets costs 1.2-1.5 ms less than in the LDS tile kernel; so by default
+    // (FSA_RTILE unset) only passes with dense gates, no remap and at most two
+    // lane exchanges run register-resident.  FSA_RTILE=1: every pass that can,
+    // FSA_RTILE=0: none.
+    const char* rtEnv = std::getenv("FSA_RTILE");
+    const int rtMode = !(sizeof(R) == 8) ? 0 : (rtEnv && *rtEnv) ? (rtEnv[0] == '0' ? 0 : 2) : 1;
+    const bool useRtile = rtMode != 0;
     for (size_t si = 0; si < stages.size(); si++) {
         const Stage& st = stages[si];
         if (!st.tiled) continue;
@@ -701,6 +712,33 @@ void execute_impl(State& s, const std::vector<HOp>& ops, const std::vector<Stage
             g.tileMask |= 1ULL << st.tq[i];
         }
         geoms[si] = g;
+        if constexpr (sizeof(R) == 8) {
+            if (useRtile && m == (unsigned)tile_bits_max<R>()) {
+                RTilePass rp;
+                const size_t i0 = items.size(), m0 = mats.size();
+                bool take = plan_rtile(ops, st, items, mats, rp) && rp.matBytes <= rtile_mat_lds_max();
+                if (take && rtMode == 1) {
+                    unsigned remaps = 0, lanes = 0, dense = 0;
+                    for (size_t i = rp.first; i < rp.first + rp.count; i++) {
+                        const TileOp& t = items[i];
+                        if (t.kind == OP_REMAP) remaps++;
+                        else if (t.kind == OP_MATK) dense++;
+                        else if (t.kind == OP_MAT1 && t.p[0] < 9) lanes++;
+                        else if (t.kind == OP_PAULI && (t.lmask & 63u)) lanes++;
+                    }
+                    take = remaps == 0 && dense > 0 && lanes <= 2;
+                }
+                if (take) {
+                    rpass[si] = rp;
+                    first[si] = rp.first;
+                    count[si] = rp.count;
+                    flags[si] = 32u;
+                    continue;
+                }
+                items.resize(i0);
+                mats.resize(m0);
+            }
+        }
         first[si] = items.size();
         // group consecutive sweepable ops whose mixed tile bits fit in W bits
         struct Group { bool sweep; uint32_t bits; std::vector<int> ops; };
@@ -835,7 +873,16 @@ void execute_impl(State& s, const std::vector<HOp>& ops, const std::vector<Stage
     }
     for (size_t si = 0; si < stages.size(); si++) {
         const Stage& st = stages[si];
-        if (st.tiled) {
+        if (st.tiled && (flags[si] & 32u)) {
+            if constexpr (sizeof(R) == 8) {
+                bool dense = false;
+                for (size_t i = first[si]; i < first[si] + count[si]; i++) dense = dense || items[i].kind == OP_MATK;
+                ProfScope prof(dense ? "rtile_pass_mfma" : "rtile_pass_plain", s.stream);
+                launch_rtile(s.stream, static_cast<cx<double>*>(s.amps), s.nq, geoms[si], ditems + first[si],
+                             (unsigned)count[si], dmats, rpass[si].loadMap, rpass[si].storeMap, rpass[si].matBytes);
+                hip_check(hipGetLastError(), "register tile kernel launch");
+            }
+        } else if (st.tiled) {
             // one profile name per kernel instantiation family, so the timings
             // line up with per-kernel profiler statistics
             const uint32_t var = (flags[si] >> 2) & 3u;

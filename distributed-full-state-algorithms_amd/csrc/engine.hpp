@@ -141,6 +141,19 @@ void flush(State& s);
 template <typename R>
 TileOp translate_item(const HOp& op, const int* pos, uint64_t tileMask, std::vector<cx<R>>& mats);
 
+// A register-resident pass (k_rtile, FP64 tiles of 2^12): its items in slot
+// space (with OP_REMAP items between layouts), the layouts of its load and
+// store (4 bits per slot), and the LDS bytes of its MFMA matrices.
+struct RTilePass {
+    uint64_t loadMap = 0, storeMap = 0;
+    uint32_t matBytes = 0;
+    size_t first = 0, count = 0;
+};
+// appends the pass's items and matrices; false if an op of the stage has no
+// register-resident form (the stage then runs on the LDS tile kernel)
+bool plan_rtile(const std::vector<HOp>& ops, const Stage& st, std::vector<TileOp>& items,
+                std::vector<cx<double>>& mats, RTilePass& out);
+
 // brackets launches on `stream` with HIP events while profiling is enabled
 class ProfScope {
 public:

@@ -47,6 +47,14 @@ void launch_scale(hipStream_t s, cx<R>* a, uint64_t num, R f);
 template <typename R>
 void launch_tile(hipStream_t s, cx<R>* a, unsigned nq, const TileGeom& geom, const TileOp* ditems,
                  unsigned numItems, const TileOp* dmicro, const cx<R>* dmats, uint32_t flags, uint32_t matLds = 0);
+// register-resident pass over FP64 tiles of 2^12 amplitudes (kernels_rtile.hip):
+// items in slot space, loadMap / storeMap the slot layouts of the load and
+// the store, matLds the LDS bytes of the pass's MFMA matrices
+void launch_rtile(hipStream_t s, cx<double>* a, unsigned nq, const TileGeom& geom, const TileOp* ditems,
+                  unsigned numItems, const cx<double>* dmats, uint64_t loadMap, uint64_t storeMap, uint32_t matLds);
+constexpr uint32_t RTILE_MAT_LDS_MAX = 163840 - 2 * 65536;   // one workgroup per CU: LDS beyond the two remap halves
+bool rtile_prefetch();              // the pass kernel keeps the next tile in registers (one workgroup per CU)
+uint32_t rtile_mat_lds_max();       // LDS bytes a register-resident pass has for MFMA matrices
 // largest tile (log2 amplitudes) the LDS budget allows for this precision
 template <typename R> constexpr int tile_bits_max() { return sizeof(R) == 8 ? 12 : 13; }
 // largest tile a pass may use when asked for (FSA_TILE_BITS): 128 KiB of LDS

@@ -30,6 +30,7 @@
 #define __shared__ static
 #define __builtin_amdgcn_readfirstlane(x) (x)
 #define __builtin_amdgcn_s_waitcnt(x) ((void)(x))   // memory operations complete at once here
+#define __builtin_amdgcn_sched_barrier(x) ((void)(x))   // instruction scheduling only
 #define HIP_SYMBOL(x) (&x)
 
 struct dim3 {

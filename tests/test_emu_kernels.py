@@ -48,18 +48,19 @@ def _build():
     BUILD.mkdir(exist_ok=True)
     exe = BUILD / "emu_drive"
     sources = [CSRC / f for f in ("kernels_tile.hip", "kernels_stream.hip", "kernels_batch.hip", "kernels_reduce.hip",
-                                  "engine.cpp",
+                                  "kernels_rtile.hip", "rtile_plan.cpp", "engine.cpp",
                                   "circuit.cpp", "batch.cpp", "engine.hpp", "tile_ops.hpp", "common.hpp", "launch.hpp")]
     sources += [EMU / "emu_rt.cpp", EMU / "emu_drive.cpp", EMU / "hip" / "hip_runtime.h"]
     if exe.exists() and all(s.stat().st_mtime < exe.stat().st_mtime for s in sources):
         return exe
     # dynamic LDS becomes a static array (workgroups run one at a time)
-    for name in ("kernels_tile", "kernels_stream"):
+    for name in ("kernels_tile", "kernels_stream", "kernels_rtile"):
         src = (CSRC / f"{name}.hip").read_text().replace(
             "extern __shared__ unsigned char smem_raw[];",
             "static unsigned char smem_raw[262144] __attribute__((aligned(16)));")
         (BUILD / f"{name}_emu.cpp").write_text(src)
     units = [(BUILD / "kernels_tile_emu.cpp", []), (BUILD / "kernels_stream_emu.cpp", []),
+             (BUILD / "kernels_rtile_emu.cpp", []), (CSRC / "rtile_plan.cpp", []),
              (CSRC / "kernels_batch.hip", ["-x", "c++"]), (CSRC / "kernels_reduce.hip", ["-x", "c++"]),
              (CSRC / "engine.cpp", []), (CSRC / "circuit.cpp", []), (CSRC / "batch.cpp", []),
              (EMU / "emu_rt.cpp", []), (EMU / "emu_drive.cpp", [])]
@@ -83,7 +84,16 @@ def _build():
 def drive():
     if not (Path(CXX).exists() or shutil.which(CXX)):
         pytest.skip("ROCm clang++ not installed")
-    return _build()
+    import fcntl
+
+    BUILD.mkdir(exist_ok=True)
+    # one build at a time when several test processes start together
+    with open(BUILD / ".lock", "w") as lock:
+        fcntl.flock(lock, fcntl.LOCK_EX)
+        try:
+            return _build()
+        finally:
+            fcntl.flock(lock, fcntl.LOCK_UN)
 
 
 def _run(exe, args, env=None):
@@ -186,7 +196,8 @@ def test_mfma_pass_with_matrices_in_lds_on_cpu(drive, orc, tmp_path, bits, ks):
         ops.append(("oneTargGate", (q[0], _unitary(rng, 2))))
     path = tmp_path / "mfma.bin"
     _circuit_file(path, orc, n, ops, rng)
-    out = _run(drive, ["circuit", path], {"FSA_TILE_BITS": bits})
+    # the LDS tile kernel (a remap-free dense pass may otherwise run register-resident)
+    out = _run(drive, ["circuit", path], {"FSA_TILE_BITS": bits, "FSA_RTILE": "0"})
     assert out["rel_err"] <= 1e-12, out
     assert out.get("tile_pass_mfma", 0) == 1 and out["passes"] == 1, out
 
@@ -322,3 +333,61 @@ def test_standalone_dense_gate_between_passes_on_cpu(drive, orc, tmp_path):
     _circuit_file(path, orc, n, ops, rng)
     out = _run(drive, ["circuit", path])
     assert out["rel_err"] <= 1e-12 and out["passes"] == 3 and out["tiled"] == 2, out
+
+
+# ---- register-resident passes (kernels_rtile.hip, rtile_plan.cpp) ---------------
+
+@pytest.mark.parametrize("seed", [61, 62, 63, 64])
+def test_register_resident_passes_on_cpu(drive, orc, tmp_path, seed):
+    # random circuits on 12-bit FP64 tiles: one-qubit gates on lane and register
+    # slots, controls anywhere, Pauli strings, phase gadgets and dense gates of
+    # 2-5 targets on MFMA with operands from registers, with remaps between
+    # layouts; swaps become relabelings whose fix-up passes run on the LDS kernel
+    rng = np.random.default_rng(seed)
+    n = 15
+    ops = [_random_op(rng, n) for _ in range(70)]
+    path = tmp_path / "rt.bin"
+    _circuit_file(path, orc, n, ops, rng)
+    out = _run(drive, ["circuit", path], {"FSA_RTILE": "1"})
+    assert out["rel_err"] <= 1e-12, out
+    assert out.get("rtile_pass_mfma", 0) + out.get("rtile_pass_plain", 0) >= 2, out
+
+
+def test_register_resident_remaps_on_cpu(drive, orc, tmp_path):
+    # dense gates on qubits 0-2 (they leave the lane slots the HBM accesses need,
+    # so the pass ends with a remap), Pauli strings over 9 qubits (X/Y on wave
+    # slots), controls on wave slots, gates of every size in one pass
+    rng = np.random.default_rng(65)
+    n = 14
+    ops = [("manyTargGate", ([0, 7, 2, 11, 5], _unitary(rng, 32))),
+           ("oneTargGate", (9, _unitary(rng, 2))),
+           ("pauliGadget", ([0, 1, 2, 3, 5, 7, 9, 11, 13], [1, 2, 1, 2, 1, 2, 1, 2, 3], 0.37)),
+           ("manyCtrlOneTargGate", ([13, 12, 3], 6, _unitary(rng, 2))),
+           ("manyTargGate", ([1, 12], _unitary(rng, 4))),
+           ("manyTargGate", ([13, 0, 6], _unitary(rng, 8))),
+           ("phaseGadget", ([1, 4, 8, 12], -0.8)),
+           ("manyTargGate", ([3, 6, 9, 12], _unitary(rng, 16))),
+           ("pauliTensor", ([2, 4, 6, 8, 10, 12], [1, 1, 2, 2, 3, 1])),
+           ("oneTargGate", (1, _unitary(rng, 2)))]
+    path = tmp_path / "remap.bin"
+    _circuit_file(path, orc, n, ops, rng)
+    out = _run(drive, ["circuit", path], {"FSA_RTILE": "1"})
+    assert out["rel_err"] <= 1e-12, out
+    assert out.get("rtile_pass_mfma", 0) >= 1, out
+
+
+def test_register_resident_auto_choice_on_cpu(drive, orc, tmp_path):
+    # by default a dense pass that needs no remap runs register-resident and a
+    # pass that would need remaps runs on the LDS tile kernel
+    rng = np.random.default_rng(66)
+    n = 14
+    dense = [("manyTargGate", ([4, 9, 12], _unitary(rng, 8))), ("oneTargGate", (10, _unitary(rng, 2)))]
+    path = tmp_path / "auto_dense.bin"
+    _circuit_file(path, orc, n, dense, rng)
+    out = _run(drive, ["circuit", path])
+    assert out["rel_err"] <= 1e-12 and out.get("rtile_pass_mfma", 0) == 1, out
+    wide = [("oneTargGate", (q, _unitary(rng, 2))) for q in (3, 4, 5, 6, 7, 8, 9, 10, 13)]
+    path = tmp_path / "auto_wide.bin"
+    _circuit_file(path, orc, n, wide, rng)
+    out = _run(drive, ["circuit", path])
+    assert out["rel_err"] <= 1e-12 and out.get("tile_pass_plain", 0) == 1, out

@@ -101,6 +101,7 @@ def test_sv_one_target_every_qubit_eager(fsa, orc, n):
 
 
 TUNINGS = {"default": {}, "valu-dense": {"FSA_MFMA": "0"}, "sweeps": {"FSA_SWEEPS": "1"},
+           "register-tiles": {"FSA_RTILE": "1"}, "lds-tiles": {"FSA_RTILE": "0"},
            "sweeps-valu": {"FSA_SWEEPS": "1", "FSA_MFMA": "0"}, "tile13": {"FSA_TILE_BITS": "13"},
            "tile13-low5": {"FSA_TILE_BITS": "13", "FSA_TILE_LOW": "5"}}
 
@@ -125,7 +126,7 @@ def test_sv_fused_circuits(fsa, orc, n, tuning, monkeypatch):
 
 
 @pytest.mark.parametrize("n", [14, 17])
-@pytest.mark.parametrize("tuning", ["default", "valu-dense", "tile13"])
+@pytest.mark.parametrize("tuning", ["default", "valu-dense", "tile13", "register-tiles"])
 def test_dense_only_circuits(fsa, orc, n, tuning, monkeypatch):
     """Passes holding only 3-5 target dense gates (the MFMA form on full tiles of
     2^12 and 2^13 amplitudes, the VALU form otherwise), unitary matrices."""
