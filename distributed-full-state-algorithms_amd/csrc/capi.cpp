@@ -681,6 +681,48 @@ int fsa_densitymatrix_partialTrace(fsa_state* h, const unsigned* targets, unsign
     });
 }
 
+// ---- the reference's local subroutines ------------------------------------------------
+
+int fsa_local_statevector_pauliTensorOrGadget(fsa_state* h, uint64_t maskXY, uint64_t maskYZ, const double* powI,
+                                              const double* thisAmpFac, const double* otherAmpFac) {
+    return guard([&] {
+        State& s = sv(h);
+        need(powI && thisAmpFac && otherAmpFac, FSA_ERR_ARG, "null factor");
+        const uint64_t inRange = s.N >= 64 ? ~0ULL : (1ULL << s.N) - 1;
+        need(!(maskXY & ~inRange) && !(maskYZ & ~inRange), FSA_ERR_QUBIT, "Pauli mask names a qubit beyond the state");
+        // local_statevector.hpp:126-131: new[j0] = this*a0 + other*powI*(-1)^{par(j1 & maskYZ)}*a1
+        HOp o;
+        o.kind = OP_PAULI;
+        o.mxy = maskXY;
+        o.myz = maskYZ;
+        o.pw = cd(powI[0], powI[1]);
+        o.th = cd(thisAmpFac[0], thisAmpFac[1]);
+        o.fe = cd(otherAmpFac[0], otherAmpFac[1]) * o.pw;
+        finish_op(o);
+        submit(s, {std::move(o)});
+    });
+}
+
+int fsa_local_densitymatrix_oneQubitDepolarising(fsa_state* h, unsigned qb, double c1, double c2, double c3) {
+    return guard([&] {
+        State& s = dmx(h);
+        check_qubits(&qb, 1, s.N, "qubit");
+        need(std::isfinite(c1) && std::isfinite(c2) && std::isfinite(c3), FSA_ERR_ARG, "coefficients must be finite");
+        submit(s, {op_chan(OP_DEPOL1, {qb, qb + s.N}, c1, c2, c3)});
+    });
+}
+
+int fsa_local_densitymatrix_twoQubitDepolarising(fsa_state* h, unsigned q0, unsigned q1, unsigned q2, unsigned q3,
+                                                 double c1, double c2, double c3) {
+    return guard([&] {
+        State& s = dmx(h);
+        const unsigned q[4] = {q0, q1, q2, q3};
+        check_qubits(q, 4, s.nq, "qubit");
+        need(std::isfinite(c1) && std::isfinite(c2) && std::isfinite(c3), FSA_ERR_ARG, "coefficients must be finite");
+        submit(s, {op_chan(OP_DEPOL2, {q0, q1, q2, q3}, c1, c2, c3)});
+    });
+}
+
 // ---- extensions -----------------------------------------------------------------------
 
 int fsa_statevector_expecPauliSum(fsa_state* h, const double* coeffs, unsigned numTerms, const unsigned* paulis,

@@ -508,7 +508,7 @@ void run_standalone(State& s, const HOp& op) {
     case OP_DEPHASE2: launch_dephase2<R>(s.stream, a, N, op.q[0], op.q[2], (R)op.c[0]); break;
     case OP_DEPOL1: launch_depol1<R>(s.stream, a, N, op.q[0], (R)op.c[0], (R)op.c[1], (R)op.c[2]); break;
     case OP_DEPOL2:
-        launch_depol2<R>(s.stream, a, N, op.q[0], op.q[1], (R)op.c[0], (R)op.c[1], (R)op.c[2]);
+        launch_depol2<R>(s.stream, a, nq, op.q.data(), (R)op.c[0], (R)op.c[1], (R)op.c[2]);
         break;
     case OP_DAMP: launch_damp<R>(s.stream, a, N, op.q[0], (R)op.c[0], (R)op.c[1], (R)op.c[2]); break;
     }

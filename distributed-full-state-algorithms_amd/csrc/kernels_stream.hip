@@ -293,7 +293,8 @@ __global__ __launch_bounds__(BLOCK) void k_depol1(cx<R>* __restrict__ a, uint64_
     }
 }
 
-// :83-108 in ONE pass: per group of 16 (bits q1, q2, q1+N, q2+N) scale the 12
+// :83-108 in ONE pass: per group of 16 (bits q0..q3; the channel passes q1, q2,
+// q1+N, q2+N) scale the 12
 // off-block-diagonal entries by 1 + c3 and mix the 4 block-diagonal ones as
 // c1*x + c2*sum.
 template <typename R>
@@ -449,9 +450,9 @@ void launch_depol1(hipStream_t s, cx<R>* a, unsigned N, unsigned q, R c1, R c2, 
 }
 
 template <typename R>
-void launch_depol2(hipStream_t s, cx<R>* a, unsigned N, unsigned q1, unsigned q2, R c1, R c2, R c3) {
-    uint64_t numIts = (1ULL << (2 * N)) >> 4;
-    uint64_t b0 = 1ULL << q1, b1 = 1ULL << q2, b2 = 1ULL << (q1 + N), b3 = 1ULL << (q2 + N);
+void launch_depol2(hipStream_t s, cx<R>* a, unsigned nq, const unsigned* q, R c1, R c2, R c3) {
+    uint64_t numIts = (1ULL << nq) >> 4;
+    uint64_t b0 = 1ULL << q[0], b1 = 1ULL << q[1], b2 = 1ULL << q[2], b3 = 1ULL << q[3];
     hipLaunchKernelGGL(k_depol2<R>, dim3((unsigned)((numIts + BLOCK - 1) / BLOCK)), dim3(BLOCK), 0, s, a,
                        numIts, b0 | b1 | b2 | b3, b0, b1, b2, b3, c1, c2, c3);
 }
@@ -474,7 +475,7 @@ void launch_damp(hipStream_t s, cx<R>* a, unsigned N, unsigned q, R prob, R c1, 
     template void launch_dephase1<R>(hipStream_t, cx<R>*, unsigned, unsigned, R);                      \
     template void launch_dephase2<R>(hipStream_t, cx<R>*, unsigned, unsigned, unsigned, R);            \
     template void launch_depol1<R>(hipStream_t, cx<R>*, unsigned, unsigned, R, R, R);                  \
-    template void launch_depol2<R>(hipStream_t, cx<R>*, unsigned, unsigned, unsigned, R, R, R);        \
+    template void launch_depol2<R>(hipStream_t, cx<R>*, unsigned, const unsigned*, R, R, R);             \
     template void launch_damp<R>(hipStream_t, cx<R>*, unsigned, unsigned, R, R, R);
 
 FSA_INSTANTIATE(double)

@@ -30,7 +30,7 @@ void launch_dephase2(hipStream_t s, cx<R>* a, unsigned N, unsigned q1, unsigned 
 template <typename R>
 void launch_depol1(hipStream_t s, cx<R>* a, unsigned N, unsigned q, R c1, R c2, R c3);
 template <typename R>
-void launch_depol2(hipStream_t s, cx<R>* a, unsigned N, unsigned q1, unsigned q2, R c1, R c2, R c3);
+void launch_depol2(hipStream_t s, cx<R>* a, unsigned nq, const unsigned* q /* [4] */, R c1, R c2, R c3);
 template <typename R>
 void launch_damp(hipStream_t s, cx<R>* a, unsigned N, unsigned q, R prob, R c1, R c2);
 template <typename R>

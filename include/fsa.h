@@ -137,6 +137,23 @@ int fsa_densitymatrix_expecPauliString(fsa_state* rho, const double* coeffs, uns
 int fsa_densitymatrix_partialTrace(fsa_state* rho, const unsigned* targets, unsigned numTargets,
                                    fsa_state** out);                              /* :347-406 */
 
+/* ---------------------------------------------------------------------------
+ * the reference's local subroutines whose arguments are not those of a public
+ * operation (at one rank the local amplitudes are the whole state)
+ * ------------------------------------------------------------------------- */
+int fsa_local_statevector_pauliTensorOrGadget(fsa_state* psi, uint64_t maskXY, uint64_t maskYZ,
+                                              const double* powI /* [2] */, const double* thisAmpFac /* [2] */,
+                                              const double* otherAmpFac /* [2] */);
+    /* local_statevector_pauliTensorOrGadget_subroutine, local_statevector.hpp:102-135:
+     * psi <- thisAmpFac psi + otherAmpFac powI X^maskXY Z^maskYZ psi (Z applied first) */
+int fsa_local_densitymatrix_oneQubitDepolarising(fsa_state* rho, unsigned qb, double c1, double c2, double c3);
+    /* local_densitymatrix.hpp:63-81 with the caller's coefficients (real, as every caller passes) */
+int fsa_local_densitymatrix_twoQubitDepolarising(fsa_state* rho, unsigned q0, unsigned q1, unsigned q2,
+                                                 unsigned q3, double c1, double c2, double c3);
+    /* local_densitymatrix.hpp:83-108 on any four distinct bits of the 2N-bit index: amplitudes
+     * with bit q0 == bit q2 and bit q1 == bit q3 mix (c1 amp + c2 sum of the four), the others
+     * scale by 1 + c3 */
+
 /* No counterpart in the reference: fsa_densitymatrix_expecPauliString for
  * callers that cannot allocate (code under HIP graph capture, serving loops,
  * harnesses that forbid hipMalloc).  Same arguments, same kernels, same result

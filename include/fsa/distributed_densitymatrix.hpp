@@ -6,6 +6,7 @@
 #define DISTRIBUTED_DENSITYMATRIX_HPP
 
 #include "distributed_statevector.hpp"
+#include "local_densitymatrix.hpp"
 
 static void distributed_densitymatrix_manyTargGate(DensityMatrix& rho, NatArray targets, AmpMatrix gate) { // :15-25
     fsa_check(fsa_densitymatrix_manyTargGate(rho.handle, targets.data(), (unsigned)targets.size(),

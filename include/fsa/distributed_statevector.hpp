@@ -4,25 +4,10 @@
 #ifndef DISTRIBUTED_STATEVECTOR_HPP
 #define DISTRIBUTED_STATEVECTOR_HPP
 
-#include <vector>
-
-#include "fsa_check.hpp"
+#include "local_statevector.hpp"
 #include "misc.hpp"
-#include "states.hpp"
-#include "types.hpp"
 
-namespace fsa_dropin {
-// row-major interleaved (re, im) copy of a square AmpMatrix
-inline std::vector<double> flat(const AmpMatrix& m) {
-    std::vector<double> f;
-    f.reserve(2 * m.size() * m.size());
-    for (const AmpArray& row : m)
-        for (const Amp& v : row) { f.push_back(v.real()); f.push_back(v.imag()); }
-    return f;
-}
-}  // namespace fsa_dropin
-
-inline void distributed_statevector_oneTargGate(StateVector& psi, Nat target, AmpMatrix gate) {      // :18-40
+static void distributed_statevector_oneTargGate(StateVector& psi, Nat target, AmpMatrix gate) {      // :18-40
     fsa_check(fsa_statevector_oneTargGate(psi.handle, target, fsa_dropin::flat(gate).data()), "oneTargGate");
 }
 

@@ -49,6 +49,10 @@ public:
         syncFromDevice();
         return amps;
     }
+    // states.hpp:27-30; defined in test_utilities.hpp, as in the reference
+    void setRandomAmps();
+    void printAmps();
+    bool agreesWith(AmpArray allAmps, Real tol = 1E-5);
     void syncFromDevice() {
         amps.resize(numAmpsPerNode);
         fsa_check(fsa_state_download(handle, reinterpret_cast<double*>(amps.data()), numAmpsPerNode), "download");
@@ -123,6 +127,8 @@ public:
             for (Index c = 0; c < dim; c++) m[r][c] = v[dim * c + r];
         return m;
     }
+    // states.hpp:54-55; defined in test_utilities.hpp
+    bool agreesWith(AmpMatrix allAmps, Real tol = 1E-5);
 
 private:
     explicit DensityMatrix(fsa_state* h) : StateVector(h) {}

@@ -277,3 +277,5 @@ def build(quiet: bool = True) -> None:
     subprocess.run(["make", "-C", str(HERE), "-B", "liboracle.so"], check=True, stdout=out)
     if Path(os.environ.get("FSA_REFERENCE_DIR", "/root/reference"), "src").is_dir():
         subprocess.run(["make", "-C", str(HERE), "-B", "ref"], check=True, stdout=out)
+        # the reference's own test suite over the drop-in headers (needs libfsa.so)
+        subprocess.run(["make", "-C", str(HERE), "suite"], check=True, stdout=out)

@@ -11,6 +11,9 @@
 #include <string>
 
 #include "distributed_densitymatrix.hpp"
+#include "local_densitymatrix.hpp"
+#include "local_statevector.hpp"
+#include "test_utilities.hpp"
 #include "../../oracle/fsa_oracle.h"
 
 static const int TRIALS = 200;
@@ -41,6 +44,24 @@ static double relErr(const AmpArray& a, const AmpArray& b) {
     double num = 0, den = 0;
     for (size_t i = 0; i < a.size(); i++) { num += std::norm(a[i] - b[i]); den += std::norm(b[i]); }
     return std::sqrt(num / den);
+}
+
+// amplitude idx -> the index whose bit perm[k] is bit k of idx
+static AmpArray permuteBits(const AmpArray& v, const std::vector<Nat>& perm) {
+    AmpArray out(v.size());
+    for (Index i = 0; i < v.size(); i++) {
+        Index j = 0;
+        for (Nat k = 0; k < perm.size(); k++) j |= (Index)getBit(i, k) << perm[k];
+        out[j] = v[i];
+    }
+    return out;
+}
+
+static void check(const char* name, bool ok) {
+    if (!ok) {
+        if (g_failures < 20) std::printf("FAIL %s\n", name);
+        g_failures++;
+    }
 }
 
 template <class S>
@@ -190,6 +211,146 @@ int main() {
                 g_failures++;
             }
         }
+    }
+    // ---- local_statevector.hpp / local_densitymatrix.hpp (one rank: the whole state)
+    for (int t = 0; t < TRIALS; t++) {
+        {
+            AmpMatrix g = randMatrix(2);
+            Nat targ = rand() % NQ_SV;
+            trial<StateVector>("local_statevector_oneTargGate", psi,
+                [&](StateVector& s) { local_statevector_oneTargGate(s, targ, g); },
+                [&](double* r) { orc_sv_oneTargGate(r, NQ_SV, targ, flatm(g).data()); });
+            NatArray q = uniqueNats(NQ_SV, 3);
+            NatArray ctrls = {q[1], q[2]};
+            trial<StateVector>("local_statevector_manyCtrlOneTargGate", psi,
+                [&](StateVector& s) { local_statevector_manyCtrlOneTargGate(s, ctrls, q[0], g); },
+                [&](double* r) { orc_sv_manyCtrlOneTargGate(r, NQ_SV, ctrls.data(), 2, q[0], flatm(g).data()); });
+            trial<StateVector>("local_statevector_swapGate", psi,
+                [&](StateVector& s) { local_statevector_swapGate(s, q[0], q[2]); },
+                [&](double* r) { orc_sv_swapGate(r, NQ_SV, q[0], q[2]); });
+            AmpMatrix g3 = randMatrix(8);
+            trial<StateVector>("local_statevector_manyTargGate", psi,
+                [&](StateVector& s) { local_statevector_manyTargGate(s, q, g3); },
+                [&](double* r) { orc_sv_manyTargGate(r, NQ_SV, q.data(), 3, flatm(g3).data()); });
+            Real th = -3.14159 + 6.28318 * urand();
+            trial<StateVector>("local_statevector_phaseGadget", psi,
+                [&](StateVector& s) { local_statevector_phaseGadget(s, q, th); },
+                [&](double* r) { orc_sv_phaseGadget(r, NQ_SV, q.data(), 3, th); });
+        }
+        {
+            // a psi + b P psi for the Pauli tensor P (powI = i^{#Y}, local_statevector.hpp:102-135),
+            // by linearity from the oracle's pauliTensor
+            Nat k = 1 + rand() % NQ_SV;
+            NatArray targs = uniqueNats(NQ_SV, k), paulis(k);
+            Index mxy = 0, myz = 0;
+            Amp powI = 1;
+            for (Nat i = 0; i < k; i++) {
+                paulis[i] = 1 + rand() % 3;
+                if (paulis[i] != 3) mxy |= powerOf2(targs[i]);
+                if (paulis[i] != 1) myz |= powerOf2(targs[i]);
+                if (paulis[i] == 2) powI *= Amp(0, 1);
+            }
+            Amp a = grand(), b = grand();
+            trial<StateVector>("local_statevector_pauliTensorOrGadget_subroutine", psi,
+                [&](StateVector& s) {
+                    local_statevector_pauliTensorOrGadget_subroutine(s, targs, powI, mxy, myz, a, b);
+                },
+                [&](double* r) {
+                    AmpArray in(reinterpret_cast<Amp*>(r), reinterpret_cast<Amp*>(r) + psi.numAmpsPerNode);
+                    orc_sv_pauliTensor(r, NQ_SV, targs.data(), paulis.data(), k);
+                    Amp* o = reinterpret_cast<Amp*>(r);
+                    for (Index i = 0; i < in.size(); i++) o[i] = a * in[i] + b * o[i];
+                });
+        }
+        {
+            NatArray q = uniqueNats(NQ_DM, 2);
+            Real p = 0.5 * urand();
+            // the coefficients distributed_densitymatrix.hpp:106-108 passes
+            trial<DensityMatrix>("local_densitymatrix_oneQubitDepolarising", rho,
+                [&](DensityMatrix& s) {
+                    local_densitymatrix_oneQubitDepolarising(s, q[0], 2 * p / 3, 1 - 2 * p / 3, 1 - 4 * p / 3);
+                },
+                [&](double* r) { orc_dm_oneQubitDepolarising(r, NQ_DM, q[0], p); });
+            // :247-256: sorted qubits, shipped constants
+            Nat lo = std::min(q[0], q[1]), hi = std::max(q[0], q[1]);
+            trial<DensityMatrix>("local_densitymatrix_twoQubitDepolarising", rho,
+                [&](DensityMatrix& s) {
+                    local_densitymatrix_twoQubitDepolarising(s, lo, hi, lo + NQ_DM, hi + NQ_DM, 1 - 4 * p / 5,
+                                                             4 * p / 15, -16 * p / 15);
+                },
+                [&](double* r) { orc_dm_twoQubitDepolarising_asShipped(r, NQ_DM, lo, hi, p); });
+            // any four bits: relabel them onto (lo, hi, lo + N, hi + N), apply, relabel back
+            NatArray bits = uniqueNats(2 * NQ_DM, 4);
+            const Nat dest[4] = {lo, hi, lo + NQ_DM, hi + NQ_DM};
+            std::vector<Nat> perm(2 * NQ_DM, 2 * NQ_DM), inv(2 * NQ_DM);
+            std::vector<bool> used(2 * NQ_DM, false);
+            for (int i = 0; i < 4; i++) { perm[bits[i]] = dest[i]; used[dest[i]] = true; }
+            Nat nextFree = 0;
+            for (Nat b2 = 0; b2 < 2 * NQ_DM; b2++)
+                if (perm[b2] == 2 * NQ_DM) {
+                    while (used[nextFree]) nextFree++;
+                    perm[b2] = nextFree;
+                    used[nextFree] = true;
+                }
+            for (Nat b2 = 0; b2 < 2 * NQ_DM; b2++) inv[perm[b2]] = b2;
+            trial<DensityMatrix>("local_densitymatrix_twoQubitDepolarising(any bits)", rho,
+                [&](DensityMatrix& s) {
+                    local_densitymatrix_twoQubitDepolarising(s, bits[0], bits[1], bits[2], bits[3], 1 - 4 * p / 5,
+                                                             4 * p / 15, -16 * p / 15);
+                },
+                [&](double* r) {
+                    Amp* o = reinterpret_cast<Amp*>(r);
+                    AmpArray v = permuteBits(AmpArray(o, o + rho.numAmpsPerNode), perm);
+                    orc_dm_twoQubitDepolarising_asShipped(reinterpret_cast<double*>(v.data()), NQ_DM, lo, hi, p);
+                    v = permuteBits(v, inv);
+                    std::copy(v.begin(), v.end(), o);
+                });
+            trial<DensityMatrix>("local_densitymatrix_oneQubitDephasing", rho,
+                [&](DensityMatrix& s) { local_densitymatrix_oneQubitDephasing(s, q[0], p); },
+                [&](double* r) { orc_dm_oneQubitDephasing(r, NQ_DM, q[0], p); });
+            trial<DensityMatrix>("local_densitymatrix_twoQubitDephasing", rho,
+                [&](DensityMatrix& s) { local_densitymatrix_twoQubitDephasing(s, q[0], q[1], p); },
+                [&](double* r) { orc_dm_twoQubitDephasing(r, NQ_DM, q[0], q[1], p); });
+            trial<DensityMatrix>("local_densitymatrix_damping", rho,
+                [&](DensityMatrix& s) { local_densitymatrix_damping(s, q[0], p); },
+                [&](double* r) { orc_dm_damping(r, NQ_DM, q[0], p); });
+            // :134-164 with the pairing distributed_densitymatrix.hpp:365-369 uses
+            for (auto& v : rho.amps) v = grand();
+            rho.syncToDevice();
+            Nat k = 1 + rand() % NQ_DM;
+            NatArray targs = uniqueNats(NQ_DM, k), pairs = targs;
+            for (Nat& x : pairs) x += NQ_DM;
+            DensityMatrix out = local_densitymatrix_partialTrace(rho, targs, pairs);
+            AmpArray ref(out.numAmpsPerNode);
+            orc_dm_partialTrace(reinterpret_cast<double*>(rho.amps.data()), NQ_DM, targs.data(), k,
+                                reinterpret_cast<double*>(ref.data()));
+            check("local_densitymatrix_partialTrace", relErr(out.getAllVecAmps(), ref) <= 1e-12);
+        }
+    }
+
+    // ---- the test-utility members (test_utilities.hpp:437-524)
+    {
+        srand(7);
+        psi.setRandomAmps();
+        srand(7);
+        AmpArray drawn = getRandomArray(psi.numAmpsPerNode);
+        AmpArray held = psi.getAllVecAmps();
+        check("setRandomAmps draws getRandomArray's amplitudes", held == drawn);
+        check("agreesWith itself", psi.agreesWith(drawn));
+        AmpArray off = drawn;
+        off[3] -= Amp(2e-5, 0);
+        check("agreesWith flags a negative difference", !psi.agreesWith(off));
+        off = drawn;
+        off[5] += Amp(0, 2e-5);
+        check("agreesWith flags a positive difference", !psi.agreesWith(off));
+        off = drawn;
+        off[5] += Amp(0, 2e-5);
+        check("agreesWith honours tol", psi.agreesWith(off, 1e-4));
+        rho.setRandomAmps();
+        AmpMatrix m = rho.getAllMatrAmps();
+        check("DensityMatrix::agreesWith itself", rho.agreesWith(m));
+        m[1][2] -= Amp(0, 1e-3);
+        check("DensityMatrix::agreesWith flags a difference", !rho.agreesWith(m));
     }
     comm_end();
     std::printf("drop-in tests: %d trials per op, %d failures\n", TRIALS, g_failures);
