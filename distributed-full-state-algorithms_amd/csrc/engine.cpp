@@ -58,6 +58,12 @@ static unsigned env_u(const char* name, unsigned dflt) {
     return (unsigned)std::strtoul(v, nullptr, 10);
 }
 
+// Dense gates of at least this many targets (6-12) run as the MFMA GEMM of
+// kernels_gemm.hip; FSA_GEMM_MINK=13 keeps them on the per-group kernel.  At 4
+// and 5 targets the per-group kernel is faster (28 qubits, FP64: 1.70 vs 2.43
+// ms at k = 4), so the GEMM starts at 6.
+static unsigned gemm_min_k() { return std::max(6u, env_u("FSA_GEMM_MINK", 6)); }
+
 // Tuning switches (diagnostics): FSA_MFMA=0 runs dense k=3..5 gates on the
 // VALU form; FSA_SWEEPS=1 enables register sweeps for passes without dense
 // gates.  Measured on a 30-qubit FP64 circuit of 1000 gates the defaults
@@ -475,7 +481,15 @@ void run_standalone(State& s, const HOp& op) {
     case OP_MATK: {
         const unsigned k = (unsigned)op.q.size();
         const uint64_t D = 1ULL << k;
-        if (k <= 5) {
+        if (k >= gemm_min_k() && gemm_fits(nq, k)) {
+            const GemmGeom g = gemm_geom(nq, op.q.data(), k, sizeof(R));
+            std::vector<GemmFrag<R>> frag(D * D);
+            gemm_pack_matrix<R>(op.mat.data(), k, frag.data());
+            auto* df = static_cast<const GemmFrag<R>*>(s.ring.push(frag.data(), frag.size() * sizeof(GemmFrag<R>)));
+            const uint64_t se = gemm_scratch_elems(g, nq, sizeof(R));
+            cx<R>* scratch = se ? reinterpret_cast<cx<R>*>(s.workspace((se * sizeof(cx<R>) + 7) / 8)) : nullptr;
+            launch_matk_gemm<R>(s.stream, a, nq, g, df, scratch);
+        } else if (k <= 5) {
             std::vector<cx<R>> m(D * D);
             for (uint64_t i = 0; i < D * D; i++) m[i] = to_cx<R>(op.mat[i]);
             auto* dm = static_cast<const cx<R>*>(s.ring.push(m.data(), m.size() * sizeof(cx<R>)));

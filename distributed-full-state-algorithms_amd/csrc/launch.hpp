@@ -3,6 +3,9 @@
 
 #include "common.hpp"
 
+#include <algorithm>
+#include <complex>
+
 namespace fsa {
 
 constexpr int BLOCK = 256;
@@ -18,6 +21,34 @@ void launch_swap(hipStream_t s, cx<R>* a, unsigned nq, unsigned q1, unsigned q2)
 template <typename R>
 void launch_matk(hipStream_t s, cx<R>* a, unsigned nq, const unsigned* targets, unsigned k,
                  const cx<R>* dmat, const cx<R>* dmatT);
+
+// dense k-target gate, 6 <= k <= 12, as an MFMA GEMM over column blocks of
+// groups (kernels_gemm.hip).  The matrix is packed in MFMA fragment order
+// (gemm_pack_matrix); gates whose row panels span several workgroups need a
+// scratch area of gemm_scratch_elems amplitudes.
+template <typename R> struct alignas(2 * sizeof(R)) GemmFrag { R re, im; };
+constexpr uint32_t GEMM_CHUNK = 4096;   // amplitudes a workgroup stages at a time
+struct GemmGeom {
+    uint64_t fixmask;          // target bits and column bits (the column block index skips them)
+    uint32_t k, kcBits, cbits; // targets; log2 rows per staged chunk; log2 columns per block
+    uint32_t rowUnits, rbw;    // waves along the rows; 16-row blocks per wave
+    uint32_t panels;           // workgroups per column block (row panels)
+    uint32_t inplace;          // one panel and one chunk: results go straight back
+    uint8_t tq[12];            // targets in matrix order (bit i of a row index is qubit tq[i])
+    uint8_t lpos[16];          // chunk tile bits, ascending: qubit of local bit b ...
+    uint32_t lw[16];           // ... and its weight in the LDS slot (row * CBW + column)
+    uint8_t fpos[20];          // full tile bits (every target and the column bits), ascending
+    uint32_t fw[20];
+};
+bool gemm_fits(unsigned nq, unsigned k);
+GemmGeom gemm_geom(unsigned nq, const unsigned* targets, unsigned k, size_t realBytes);
+uint64_t gemm_scratch_elems(const GemmGeom& g, unsigned nq, size_t realBytes);
+template <typename R>
+void gemm_pack_matrix(const std::complex<double>* m, unsigned k, GemmFrag<R>* out);
+template <typename R>
+void launch_matk_gemm(hipStream_t s, cx<R>* a, unsigned nq, const GemmGeom& g, const GemmFrag<R>* dfrag,
+                      cx<R>* scratch);
+
 template <typename R>
 void launch_pauli(hipStream_t s, cx<R>* a, unsigned nq, uint64_t mxy, uint64_t myz, cx<R> thisFac,
                   cx<R> evenFac);

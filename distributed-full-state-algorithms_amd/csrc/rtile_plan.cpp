@@ -255,21 +255,28 @@ bool plan_rtile(const std::vector<HOp>& ops, const Stage& st, std::vector<TileOp
     std::vector<Layout> at(n);
     std::vector<char> remap(n, 0);
     Layout load, L;
+    // a first op that cannot run with tile bits 0-2 on lane slots (a dense gate
+    // on three of them: only two MFMA slots are lane slots) runs after a remap
+    // from the identity load layout
+    Layout probe;
+    bool loadFixed = n == 0 || !window_layout(needs, 0, 0, true, nullptr, probe);
+    if (loadFixed) {
+        for (int b = 0; b < NS; b++) load.place(b, b);
+        L = load;
+    }
     size_t i = 0;
     while (i < n) {
         Layout W;
-        const size_t j = window_end(needs, i, i == 0, i == 0 ? nullptr : &L, W);
-        if (i == 0) load = W;
+        const bool loadWindow = i == 0 && !loadFixed;
+        const size_t j = window_end(needs, i, loadWindow, loadWindow ? nullptr : &L, W);
+        if (loadWindow) load = W;
         else remap[i] = 1;
         for (size_t x = i; x <= j; x++) at[x] = W;
         L = W;
         i = j + 1;
     }
-    if (n == 0) {
-        for (int b = 0; b < NS; b++) load.place(b, b);
-        L = load;
-    }
     std::vector<Layout> from(n);
+    if (n) from[0] = load;
     for (size_t x = 1; x < n; x++) from[x] = at[x - 1];
     // the store needs tile bits 0-2 on lane slots, as the load does
     Layout store = L;

@@ -1,6 +1,7 @@
 // Drives libfsa's real planner and kernel code on the CPU stand-in.
 //
-//   emu_drive circuit <file>              fused circuit (make_plan + execute: k_tile, streaming kernels)
+//   emu_drive circuit <file>              fused circuit (make_plan + execute: k_tile, streaming kernels);
+//                                         EMU_EAGER=1 runs op by op instead, EMU_FP32=1 on an FP32 state
 //   emu_drive batch <file> TB BITS LOW P  batch (fsa_batch_* + k_batch), P = 0 FP64 / 1 FP32
 //   emu_drive adjoint <file> P            fsa_adjoint_gradient from |0..0> (fused forward pass,
 //                                         reverse-sweep kernels of kernels_reduce.hip)
@@ -63,24 +64,35 @@ int run_circuit(const char* path) {
     r.get(v.data(), N);
     r.get(want.data(), N);
     std::fclose(r.f);
+    const bool eager = std::getenv("EMU_EAGER") != nullptr, fp32 = std::getenv("EMU_FP32") != nullptr;
     State s;
     s.N = n;
     s.nq = n;
-    s.prec = FSA_FP64;
-    s.mode = FSA_EXEC_FUSED;
-    s.amps = std::malloc(N * sizeof(cx<double>));
-    std::memcpy(s.amps, v.data(), N * sizeof(cx<double>));
-    Plan p = make_plan(c->c.ops, n, FSA_FP64);
+    s.prec = fp32 ? FSA_FP32 : FSA_FP64;
+    s.mode = eager ? FSA_EXEC_EAGER : FSA_EXEC_FUSED;
+    s.amps = std::malloc(N * (fp32 ? sizeof(cx<float>) : sizeof(cx<double>)));
+    if (fp32)
+        for (size_t i = 0; i < N; i++) static_cast<cx<float>*>(s.amps)[i] = mk<float>((float)v[i].x, (float)v[i].y);
+    else
+        std::memcpy(s.amps, v.data(), N * sizeof(cx<double>));
+    Plan p = make_plan(c->c.ops, n, s.prec);
     size_t tiled = 0;
     for (const Stage& st : p.stages) tiled += st.tiled;
     profile_enable(true);
-    execute(s, p);
+    if (eager) execute_now(s, c->c.ops);
+    else execute(s, p);
     // launches per kernel family (the profiler's names), e.g. tile_pass_mfma=3
     for (const ProfEntry& e : profile_read()) std::printf("%s=%llu ", e.name.c_str(), (unsigned long long)e.count);
     profile_enable(false);
     double num = 0, den = 0;
     for (size_t i = 0; i < N; i++) {
-        const cx<double> a = static_cast<cx<double>*>(s.amps)[i];
+        cx<double> a;
+        if (fp32) {
+            const cx<float> f = static_cast<cx<float>*>(s.amps)[i];
+            a = mk<double>(f.x, f.y);
+        } else {
+            a = static_cast<cx<double>*>(s.amps)[i];
+        }
         num += (a.x - want[i].x) * (a.x - want[i].x) + (a.y - want[i].y) * (a.y - want[i].y);
         den += want[i].x * want[i].x + want[i].y * want[i].y;
     }

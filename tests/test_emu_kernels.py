@@ -48,7 +48,7 @@ def _build():
     BUILD.mkdir(exist_ok=True)
     exe = BUILD / "emu_drive"
     sources = [CSRC / f for f in ("kernels_tile.hip", "kernels_stream.hip", "kernels_batch.hip", "kernels_reduce.hip",
-                                  "kernels_rtile.hip", "rtile_plan.cpp", "engine.cpp",
+                                  "kernels_rtile.hip", "kernels_gemm.hip", "rtile_plan.cpp", "engine.cpp",
                                   "circuit.cpp", "batch.cpp", "engine.hpp", "tile_ops.hpp", "common.hpp", "launch.hpp")]
     sources += [EMU / "emu_rt.cpp", EMU / "emu_drive.cpp", EMU / "hip" / "hip_runtime.h"]
     if exe.exists() and all(s.stat().st_mtime < exe.stat().st_mtime for s in sources):
@@ -62,6 +62,7 @@ def _build():
     units = [(BUILD / "kernels_tile_emu.cpp", []), (BUILD / "kernels_stream_emu.cpp", []),
              (BUILD / "kernels_rtile_emu.cpp", []), (CSRC / "rtile_plan.cpp", []),
              (CSRC / "kernels_batch.hip", ["-x", "c++"]), (CSRC / "kernels_reduce.hip", ["-x", "c++"]),
+             (CSRC / "kernels_gemm.hip", ["-x", "c++"]),
              (CSRC / "engine.cpp", []), (CSRC / "circuit.cpp", []), (CSRC / "batch.cpp", []),
              (EMU / "emu_rt.cpp", []), (EMU / "emu_drive.cpp", [])]
     procs, objs = [], []
@@ -335,7 +336,48 @@ def test_standalone_dense_gate_between_passes_on_cpu(drive, orc, tmp_path):
     assert out["rel_err"] <= 1e-12 and out["passes"] == 3 and out["tiled"] == 2, out
 
 
+@pytest.mark.parametrize("k,n,eager,fp32", [(6, 12, 1, 0), (6, 13, 0, 1), (7, 12, 1, 1), (8, 12, 0, 0),
+                                             (9, 14, 0, 1)])
+def test_dense_gate_gemm_on_cpu(drive, orc, tmp_path, k, n, eager, fp32):
+    # kernels_gemm.hip: in place for k <= 8, row panels through the scratch
+    # area above (a 1 KiB cap: one column block per batch, so several
+    # batches); targets in shuffled order so that low qubits are among them
+    rng = np.random.default_rng(70 + k)
+    q = [int(x) for x in rng.permutation(n)[:k]]
+    ops = [("manyTargGate", (q, _unitary(rng, 1 << k))), ("oneTargGate", (q[0], _unitary(rng, 2)))]
+    if k <= 8:
+        ops.append(("manyTargGate", (sorted(q)[::-1], _unitary(rng, 1 << k))))
+    path = tmp_path / "gemm.bin"
+    _circuit_file(path, orc, n, ops, rng)
+    env = {"FSA_GEMM_SCRATCH_KB": "1"}
+    if eager:
+        env["EMU_EAGER"] = "1"
+    if fp32:
+        env["EMU_FP32"] = "1"
+    out = _run(drive, ["circuit", path], env)
+    assert out["rel_err"] <= (2e-6 if fp32 else 1e-12), out
+
+
 # ---- register-resident passes (kernels_rtile.hip, rtile_plan.cpp) ---------------
+
+@pytest.mark.parametrize("rtile", ["1", ""])
+@pytest.mark.parametrize("first", [[0, 1, 14, 2, 8], [2, 0, 1]])
+def test_register_resident_dense_gate_on_low_qubits_first_on_cpu(drive, orc, tmp_path, rtile, first):
+    # a pass whose first op is a dense gate on qubits 0, 1 and 2: those must sit
+    # on lane slots for the load, but only two MFMA slots are lane slots, so
+    # the pass loads in the identity layout and remaps first (the planner once
+    # failed with "an op has no layout"); forced and automatic choice
+    rng = np.random.default_rng(517)
+    n = 16
+    ops = [("manyTargGate", (first, _unitary(rng, 1 << len(first)))),
+           ("manyTargGate", ([14, 8, 15, 6, 13], _unitary(rng, 32))),
+           ("oneTargGate", (1, _unitary(rng, 2)))]
+    path = tmp_path / "low.bin"
+    _circuit_file(path, orc, n, ops, rng)
+    out = _run(drive, ["circuit", path], {"FSA_RTILE": rtile} if rtile else {})
+    assert out["rel_err"] <= 1e-12, out
+    if rtile:
+        assert out.get("rtile_pass_mfma", 0) + out.get("rtile_pass_plain", 0) >= 1, out
 
 @pytest.mark.parametrize("seed", [61, 62, 63, 64])
 def test_register_resident_passes_on_cpu(drive, orc, tmp_path, seed):
