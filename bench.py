@@ -12,7 +12,10 @@ stderr.  Fields beyond the common contract:
   per_gate_type -- each gate type run as its own streaming kernel (eager
                    mode), algorithmic GB/s (2 x 16 B x amplitudes it changes);
   cpu_baseline  -- the CPU oracle (C99/OpenMP restatement of the reference's
-                   loops) on the first gates of the same circuit, host cores.
+                   loops) on the first gates of the same circuit, host cores;
+                   per_config: the same oracle on bounded samples of the
+                   12-qubit circuit, the 14-qubit density matrix, the 12-qubit
+                   parameter-shift gradient and the 4096-circuit batch.
 """
 from __future__ import annotations
 
@@ -97,6 +100,117 @@ def cpu_baseline(circ, n, ops, seed, budget_s=15.0):
     return {"value": round(done / dt, 3), "unit": "gates/s", "cores": orc.num_threads(), "kind": "port",
             "sample": f"first {done} gates of the same {n}-qubit circuit on the C99/OpenMP oracle "
                       f"({dt:.1f} s, {nbytes / dt / 1e9:.1f} GB/s algorithmic)"}
+
+
+def _orc():
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import oracle
+
+    return oracle.oracle()
+
+
+def _sv_value(orc, psi0, ops, coeffs, paulis, thetas=None):
+    """<psi|H|psi> after ops on a copy of psi0 (gadget angles from thetas)."""
+    psi = psi0.copy()
+    k = 0
+    for name, args in ops:
+        if thetas is not None and name in ("pauliGadget", "phaseGadget"):
+            args = args[:-1] + (thetas[k],)
+            k += 1
+        getattr(orc, "sv_" + name)(psi, *args)
+    return orc.sv_innerProduct(psi, orc.sv_applyPauliSum(psi, coeffs, paulis)).real
+
+
+def cpu_baselines_per_config(circ, jobs5=None, budget_s=6.0):
+    """The oracle on the other workloads, each on a bounded sample (seconds of
+    host time), measured or extrapolated as each entry's "sample" says."""
+    orc = _orc()
+    cores = orc.num_threads()
+    out = {}
+    # 12-qubit statevector, the whole 200-gate circuit
+    ops1 = circ.config1()
+    psi = orc.init_gaussian(12, 20261203)
+    t0 = time.perf_counter()
+    reps = 0
+    while True:
+        for name, args in ops1:
+            getattr(orc, "sv_" + name)(psi, *args)
+        reps += 1
+        if time.perf_counter() - t0 > 1.0:
+            break
+    dt = (time.perf_counter() - t0) / reps
+    out["sv12_200_gates"] = {"value": round(len(ops1) / dt, 1), "unit": "gates/s", "cores": cores, "kind": "port",
+                             "sample": f"the whole 200-gate 12-qubit circuit, {reps} runs"}
+    # 14-qubit density matrix: ops until the budget, expecPauliString over 1 and
+    # 4 strings (64 extrapolated linearly), partialTrace over 4 qubits
+    ops3, coeffs3, paulis3, tt3 = circ.config3()
+    N = 14
+    rho = orc.init_gaussian(2 * N, 20261205)
+    t0 = time.perf_counter()
+    done = 0
+    for name, args in ops3:
+        getattr(orc, "dm_" + name)(rho, *args)
+        done += 1
+        if time.perf_counter() - t0 > budget_s and done >= 3:
+            break
+    t_op = (time.perf_counter() - t0) / done
+    pa = np.asarray(paulis3, dtype=np.uint32).reshape(64, N)
+    t_e = {}
+    for T in (1, 4):
+        t0 = time.perf_counter()
+        orc.dm_expecPauliString(rho, coeffs3[:T], pa[:T].ravel())
+        t_e[T] = time.perf_counter() - t0
+    per_term = max(t_e[4] - t_e[1], 0.0) / 3
+    t_exp64 = t_e[1] + 63 * per_term
+    t0 = time.perf_counter()
+    orc.dm_partialTrace(rho, tt3)
+    t_pt = time.perf_counter() - t0
+    del rho
+    out["dm14_400_ops"] = {"value": round(1.0 / t_op, 3), "unit": "ops/s", "cores": cores, "kind": "port",
+                           "expecPauliString_64_s": round(t_exp64, 3), "partialTrace_4_s": round(t_pt, 3),
+                           "sample": f"first {done} of the 400 ops ({done * t_op:.1f} s); expec over 1 and 4 "
+                                     f"strings ({t_e[1]:.2f} s, {t_e[4]:.2f} s), 64 extrapolated linearly"}
+    # 12-qubit copy of the layered gadget circuit: parameter-shift gradient =
+    # 2 x angles evaluations of <H>; a few evaluations timed
+    ops4, coeffs4, paulis4 = circ.config4(n=12)
+    psi0 = np.zeros(1 << 12, dtype=np.complex128)
+    psi0[0] = 1
+    P = len(ops4)
+    t0 = time.perf_counter()
+    evals = 0
+    while evals < 3 or time.perf_counter() - t0 < 1.0:
+        _sv_value(orc, psi0, ops4, coeffs4, paulis4)
+        evals += 1
+    t_eval = (time.perf_counter() - t0) / evals
+    out["adjoint12_parameter_shift"] = {
+        "value": round(1.0 / ((2 * P + 1) * t_eval), 4), "unit": "gradients/s", "cores": cores, "kind": "port",
+        "sample": f"{evals} evaluations of <H> on the 12-qubit copy ({P} angles, {len(coeffs4)} strings); "
+                  f"a parameter-shift gradient is 2 x {P} + 1 of them"}
+    # batch: forward <H> of a seeded sample of jobs; each job's <H> and
+    # gradient by parameter shift costs (2 x gadgets + 1) forward evaluations
+    if jobs5 is not None:
+        rng = np.random.default_rng(5)
+        order = rng.permutation(len(jobs5))
+        est, t_meas, used = [], 0.0, 0
+        for j in order:
+            n, ops, coeffs, paulis = jobs5[j]
+            psi0 = np.zeros(1 << n, dtype=np.complex128)
+            psi0[0] = 1
+            t0 = time.perf_counter()
+            _sv_value(orc, psi0, ops, coeffs, paulis)
+            t = time.perf_counter() - t0
+            t_meas += t
+            used += 1
+            gad = sum(1 for nm, _ in ops if nm == "pauliGadget")
+            est.append((2 * gad + 1) * t)
+            if t_meas > budget_s and used >= 16:
+                break
+        total = float(np.mean(est)) * len(jobs5)
+        out["batch4096_parameter_shift"] = {
+            "value": round(len(jobs5) / total, 3), "unit": "circuits/s", "cores": cores, "kind": "port",
+            "sample": f"forward <H> of {used} random jobs ({t_meas:.1f} s); each job's gradient costs "
+                      f"2 x gadgets + 1 forward evaluations, mean scaled to {len(jobs5)} jobs"}
+    return out
 
 
 def adjoint_bench(fsa, circ, n=24, reps=3):
@@ -190,13 +304,12 @@ def dm_bench(fsa, circ, N=14, seed=20261205):
     }
 
 
-def batch_bench(fsa, circ, reps=3, sample=512):
+def batch_bench(fsa, circ, jobs, reps=3, sample=512):
     """4096 independent circuits (n log-uniform in [10, 20], 50-300 gates,
     each with its own 20-string observable): every <H> and every gadget
     gradient from one persistent-kernel graph launch, FP64 and FP32, with the
     configuration picked by the batch autotuner on a spread sample."""
     t0 = time.perf_counter()
-    jobs = circ.config5()
     circuits = [fsa.Circuit(ops) for _, ops, _, _ in jobs]
     log(f"batch: {len(jobs)} circuits generated in {time.perf_counter() - t0:.1f} s")
     ns = np.array([j[0] for j in jobs])
@@ -415,15 +528,22 @@ def main():
     if not a.no_dm:
         log("density matrix: 400 ops, expecPauliString, partialTrace")
         result["density_matrix"] = dm_bench(fsa, circ)
+    jobs5 = None
+    if not a.no_batch or not a.no_cpu_baseline:
+        t0 = time.perf_counter()
+        jobs5 = circ.config5()
+        log(f"batch: {len(jobs5)} circuits generated in {time.perf_counter() - t0:.1f} s")
     if not a.no_batch:
         log("batch: 4096 circuits with gradients")
-        result["batch"] = batch_bench(fsa, circ)
+        result["batch"] = batch_bench(fsa, circ, jobs5)
     if not a.no_per_type:
         log("per-gate-type eager timing")
         result["per_gate_type"] = per_gate_type(fsa, circ, n, ops)
     if not a.no_cpu_baseline:
         log("cpu baseline (oracle) on a bounded sample")
         result["cpu_baseline"] = cpu_baseline(circ, n, ops, seed)
+        log("cpu baselines of the other workloads")
+        result["cpu_baseline"]["per_config"] = cpu_baselines_per_config(circ, jobs5)
     print(json.dumps(result), flush=True)
 
 

@@ -186,6 +186,10 @@ class StateVector:
     def init_gaussian(self, seed: int):
         _check(lib().fsa_state_init_gaussian(self._h, _u64(seed)))
 
+    def renormalise(self):
+        """Scale to unit norm (FP64 sum on the device); statevectors only."""
+        _check(lib().fsa_statevector_renormalise(self._h))
+
     def norm2(self) -> float:
         out = _d()
         _check(lib().fsa_state_norm2(self._h, C.byref(out)))

@@ -427,6 +427,24 @@ int fsa_state_norm2(fsa_state* h, double* out) {
     });
 }
 
+int fsa_statevector_renormalise(fsa_state* h) {
+    return guard([&] {
+        State& s = sv(h);
+        flush(s);
+        // <psi|psi> summed in FP64 on the device, then every amplitude scaled by
+        // its inverse square root, both on the state's stream (no host round trip)
+        double* ws = s.workspace(reduce_workspace_doubles(s.num()) + 8);
+        if (s.prec == FSA_FP64) {
+            launch_norm2<double>(s.stream, (const cx<double>*)s.amps, s.num(), ws + 8, ws);
+            launch_scale_by_inv_sqrt<double>(s.stream, (cx<double>*)s.amps, s.num(), ws);
+        } else {
+            launch_norm2<float>(s.stream, (const cx<float>*)s.amps, s.num(), ws + 8, ws);
+            launch_scale_by_inv_sqrt<float>(s.stream, (cx<float>*)s.amps, s.num(), ws);
+        }
+        hip_check(hipGetLastError(), "renormalise");
+    });
+}
+
 // ---- statevector ops -------------------------------------------------------------
 
 int fsa_statevector_oneTargGate(fsa_state* h, unsigned target, const double* gate) {

@@ -521,7 +521,7 @@ __global__ __launch_bounds__(BLOCK) void k_gaussian(cx<R>* __restrict__ a, uint6
 template <typename R>
 __global__ __launch_bounds__(BLOCK) void k_scale_inv_sqrt(cx<R>* __restrict__ a, uint64_t num,
                                                           const double* __restrict__ norm2) {
-    const double f = 1.0 / sqrt(norm2[0]);
+    const double f = norm2[0] > 0 ? 1.0 / sqrt(norm2[0]) : 1.0;   // a zero vector stays zero
     for (uint64_t j = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; j < num; j += (uint64_t)gridDim.x * BLOCK) {
         cx<R> v = a[j];
         a[j] = mk<R>((R)((double)v.x * f), (R)((double)v.y * f));

@@ -87,6 +87,11 @@ int fsa_state_init_gaussian(fsa_state* s, uint64_t seed);
 int fsa_state_device_ptr(fsa_state* s, void** ptr);
 int fsa_state_stream(fsa_state* s, void** hipStream);
 int fsa_state_norm2(fsa_state* s, double* out);   /* sum |amp|^2 in FP64 */
+/* No reference counterpart: scale a statevector to unit norm, the norm summed
+ * in FP64 on the device (an FP32 state drifts by about 1e-7 per gate).  Queued
+ * on the state's stream after the pending operations; a zero state is left
+ * as it is. */
+int fsa_statevector_renormalise(fsa_state* psi);
 
 /* ---------------------------------------------------------------------------
  * statevector operations (reference src/distributed_statevector.hpp)
