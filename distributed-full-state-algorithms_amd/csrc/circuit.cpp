@@ -101,7 +101,7 @@ const Plan& plan_for(Circuit& c, unsigned nq, int prec) {
 }
 
 
-// a statevector scratch state on a borrowed stream
+// a statevector work state on a stream it does not own
 struct Scratch {
     State s;
     Scratch(unsigned n, int prec, hipStream_t stream) {
@@ -120,13 +120,61 @@ struct Scratch {
     }
 };
 
+}  // namespace
+
+namespace fsa {
+
+// phi and lambda of the adjoint sweep on a stream of their own, with their
+// parameter rings and workspace: allocated on the first call for a size and
+// precision and reused by the next ones (a call allocates nothing then)
+struct AdjointScratch {
+    unsigned n;
+    int prec;
+    hipStream_t stream = nullptr;
+    hipEvent_t ready = nullptr;
+    std::unique_ptr<Scratch> phi, lam;
+    AdjointScratch(unsigned n_, int prec_) : n(n_), prec(prec_) {
+        try {
+            hip_check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate");
+            hip_check(hipEventCreateWithFlags(&ready, hipEventDisableTiming), "hipEventCreate");
+            phi = std::make_unique<Scratch>(n, prec, stream);
+            lam = std::make_unique<Scratch>(n, prec, stream);
+        } catch (...) {
+            release();
+            throw;
+        }
+    }
+    ~AdjointScratch() { release(); }
+    void release() {
+        phi.reset();      // each synchronises the stream before freeing
+        lam.reset();
+        if (ready) (void)hipEventDestroy(ready);
+        if (stream) (void)hipStreamDestroy(stream);
+        ready = nullptr;
+        stream = nullptr;
+    }
+};
+
+}  // namespace fsa
+
+namespace {
+
 template <typename R>
 void adjoint_impl(Circuit& c, State& psi0, const double* coeffs, unsigned T, const unsigned* paulis, double* expec,
                   double* grads) {
     const unsigned n = psi0.N;
-    hipStream_t st = psi0.stream;
     flush(psi0);
-    Scratch phi(n, psi0.prec, st), lam(n, psi0.prec, st);
+    if (!c.adj || c.adj->n != n || c.adj->prec != psi0.prec) {
+        c.adj.reset();
+        c.adj = std::make_shared<AdjointScratch>(n, psi0.prec);
+    }
+    AdjointScratch& A = *c.adj;
+    hipStream_t st = A.stream;
+    // psi0 is read below: after the work already queued on its own stream
+    hip_check(hipEventRecord(A.ready, psi0.stream), "hipEventRecord");
+    hip_check(hipStreamWaitEvent(st, A.ready, 0), "hipStreamWaitEvent");
+    Scratch& phi = *A.phi;
+    Scratch& lam = *A.lam;
     // results: slot 0 = <phi|lambda>, slot 1 + p = <lambda|P_p|phi>, then the
     // FP64 norms <psi0|psi0> and <phi|phi> (FP32 renormalisation), then the
     // gradient rows of the sweep

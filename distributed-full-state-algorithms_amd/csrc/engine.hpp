@@ -6,6 +6,7 @@
 #include "launch.hpp"
 
 #include <complex>
+#include <memory>
 #include <new>
 #include <string>
 #include <vector>
@@ -120,6 +121,8 @@ struct State {
     double* workspace(size_t doubles);
 };
 
+struct AdjointScratch;   // circuit.cpp
+
 struct Circuit {
     std::vector<HOp> ops;
     unsigned numParams = 0;
@@ -128,6 +131,9 @@ struct Circuit {
     unsigned cachedNq = 0;
     int cachedPrec = -1;
     Plan cached;
+    // the adjoint gradient's work states (phi, lambda, their stream), kept for
+    // the next call at the same size and precision; freed with the circuit
+    std::shared_ptr<AdjointScratch> adj;
 };
 
 // run ops on the state's stream; eager: each op as its own pass(es)

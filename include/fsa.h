@@ -218,7 +218,10 @@ int fsa_circuit_num_passes(fsa_circuit* c, unsigned numQubits, int precision, un
 /* Adjoint-method gradient: psi0 is the input state (left unchanged), the
  * circuit is applied, and <H> and d<H>/d theta_p for every gadget parameter p
  * are returned (grads has fsa_circuit_num_params entries).  Reverse sweep by
- * inverse gates (no stored layers); inner products accumulate in FP64. */
+ * inverse gates (no stored layers); inner products accumulate in FP64.  The
+ * circuit keeps the sweep's two work states (twice psi0's size) for its next
+ * call at the same size and precision; fsa_circuit_destroy frees them.  A
+ * circuit must not be used from two threads at once. */
 int fsa_adjoint_gradient(fsa_circuit* c, fsa_state* psi0, const double* coeffs, unsigned numTerms,
                          const unsigned* paulis, double* expec, double* grads);
 

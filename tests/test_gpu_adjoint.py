@@ -153,6 +153,47 @@ def test_adjoint_random_circuits_vs_oracle_shift(fsa, orc, seed):
     assert rel(psi0.download(), psi0h) == 0
 
 
+def test_adjoint_reuses_its_work_states(fsa, orc):
+    """The circuit keeps the adjoint's work states for the next call: calls on
+    states of another size or precision rebuild them, repeated calls give the
+    same bits, and work queued on the input state just before a call (an eager
+    gate) is ordered before the sweep reads it."""
+    rng = np.random.default_rng(41)
+    n = 10
+    ops = random_param_circuit(rng, n, 40)
+    T = 6
+    paulis = np.zeros((T, n), dtype=np.uint32)
+    for t in range(T):
+        paulis[t, rng.choice(n, 2, replace=False)] = rng.integers(1, 4, 2)
+    coeffs = rng.normal(size=T)
+    c = fsa.Circuit(ops)
+    v = rng.normal(size=1 << n) + 1j * rng.normal(size=1 << n)
+    v /= np.linalg.norm(v)
+    first = None
+    for prec, size in (("fp64", n), ("fp32", n), ("fp64", n + 2), ("fp64", n), ("fp64", n)):
+        psi0 = fsa.StateVector(size, precision=prec, mode=0)
+        w = np.zeros(1 << size, dtype=np.complex128)
+        w[:1 << n] = v
+        psi0.upload(w)
+        e, g = fsa.adjoint_gradient(c, psi0, coeffs, np.hstack([paulis, np.zeros((T, size - n), np.uint32)]))
+        if prec == "fp64" and size == n:
+            if first is None:
+                first = (e, g.copy())
+            else:
+                assert e == first[0] and np.array_equal(g, first[1])
+        psi0.close()
+    # a gate queued on psi0 right before the call is applied before the sweep reads psi0
+    psi0 = fsa.StateVector(n, mode=0)
+    psi0.upload(v)
+    h = np.array([[1, 1], [1, -1]]) / np.sqrt(2)
+    fsa.distributed_statevector_oneTargGate(psi0, 3, h)
+    e, g = fsa.adjoint_gradient(c, psi0, coeffs, paulis)
+    ref = v.copy()
+    orc.sv_oneTargGate(ref, 3, h)
+    thetas = np.array([a[-1] for nm, a in ops if nm in ("pauliGadget", "phaseGadget")])
+    assert abs(e - oracle_value(orc, ref, ops, thetas, coeffs, paulis)) <= 1e-12
+
+
 @pytest.mark.parametrize("n", [9, 12])
 def test_adjoint_gadget_runs_vs_oracle_shift(fsa, orc, n):
     # the reverse sweep takes runs of consecutive diagonal gadgets (phase
