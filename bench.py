@@ -377,6 +377,44 @@ def micro_passes(fsa, n, reps=3):
     print(json.dumps({k: {"launches": v[0], "mean_ms": round(v[1] / v[0], 4)} for k, v in prof.items()}), flush=True)
 
 
+def dense_gates(fsa, n=28, reps=3):
+    """Dense gates of 6-12 targets on an n-qubit state (n - 2 from 9 targets),
+    eager, FP64 and FP32: the MFMA GEMM (kernels_gemm.hip) and, at the same
+    sizes, the per-group kernel it replaces (FSA_GEMM_MINK=13).  ms per gate
+    from the library's HIP events, flops 8 * 2^k * 2^n."""
+    rng = np.random.default_rng(1)
+    out = []
+    for prec in ("fp64", "fp32"):
+        for k in range(6, 13):
+            nq = n if k <= 8 else n - 2
+            d = 1 << k
+            g = rng.normal(size=(d, d)) + 1j * rng.normal(size=(d, d))
+            t = [int(x) for x in rng.permutation(nq)[:k]]
+            row = {"precision": prec, "k": k, "qubits": nq}
+            for form, mink in (("gemm", "6"), ("per_group", "13")):
+                if form == "per_group" and k >= 10:
+                    continue            # seconds per gate at these sizes
+                os.environ["FSA_GEMM_MINK"] = mink
+                psi = fsa.StateVector(nq, precision=prec, mode=fsa.FSA_EXEC_EAGER)
+                psi.init_gaussian(3)
+                fsa.distributed_statevector_manyTargGate(psi, t, g)
+                psi.sync()
+                fsa.profile_enable(True)
+                for _ in range(reps):
+                    fsa.distributed_statevector_manyTargGate(psi, t, g)
+                psi.sync()
+                prof = fsa.profile_read()
+                fsa.profile_enable(False)
+                psi.close()
+                ms = prof["matk"][1] / prof["matk"][0]
+                row[form + "_ms"] = round(ms, 3)
+                row[form + "_TFLOP_per_s"] = round(8 * (1 << (nq + k)) / (ms * 1e-3) / 1e12, 2)
+            os.environ.pop("FSA_GEMM_MINK", None)
+            out.append(row)
+            print(json.dumps(row), flush=True)
+    return out
+
+
 def copy_calibration(n, reps=5):
     """A plain device-to-device copy of one n-qubit FP64 state (torch copy_),
     timed with events: the practical streaming ceiling next to the 8 TB/s peak,
@@ -416,6 +454,8 @@ def main():
                     help="only time a plain copy of one state-sized buffer and exit")
     ap.add_argument("--micro-passes", action="store_true",
                     help="only run one dense-gate pass and one diagonal pass (3 launches each) and exit")
+    ap.add_argument("--dense-gates", action="store_true",
+                    help="only time dense gates of 6-12 targets (GEMM and per-group kernels) and exit")
     a = ap.parse_args()
 
     if a.gpus != 1:
@@ -430,6 +470,9 @@ def main():
     fsa = ge.load_package()
     if a.micro_passes:
         micro_passes(fsa, a.qubits)
+        return
+    if a.dense_gates:
+        dense_gates(fsa, min(a.qubits, 28))
         return
     circ = load_circuits_module()
     n = a.qubits

@@ -233,6 +233,47 @@ def test_dm_fused_sequences(fsa, orc, N):
     assert rel(rho.download(), ref) <= TOL64
 
 
+@pytest.mark.parametrize("N", [3, 6, 9])
+@pytest.mark.parametrize("mode", MODES, ids=["eager", "fused"])
+def test_dm_fp32_against_fp64_oracle(fsa, orc, N, mode):
+    """FP32 density matrices: 40 random ops (unitary dense gates, channels, Kraus
+    maps on up to 3 targets, i.e. the 6-target GEMM), then expecPauliString and
+    partialTrace, against the FP64 oracle."""
+    rng = np.random.default_rng(900 + 10 * N + mode)
+    rho = fsa.DensityMatrix(N, precision="fp32", mode=mode)
+    v = rs(rng, 2 * N)
+    rho.upload(v)
+    ref = v.copy()
+    for _ in range(40):
+        name, args = random_dm_op(rng, N)
+        if name == "manyTargGate":
+            u, _, vh = np.linalg.svd(args[1])
+            args = (args[0], u @ vh)
+        if name == "krausMap" and N >= 3 and rng.random() < 0.5:
+            q = [int(x) for x in rng.permutation(N)[:3]]
+            ops = np.stack([unitary_like(rng, 8) / np.sqrt(2) for _ in range(2)])
+            args = (ops, q)
+        apply_dm(fsa, orc, rho, ref, name, args)
+    err = rel(rho.download(), ref)
+    print(f"fp32 dm N={N} mode={mode}: rel err {err:.3e}")
+    assert err <= 2e-6, err
+    T = 8
+    paulis = rng.integers(0, 4, T * N)
+    coeffs = rng.normal(size=T)
+    e = fsa.distributed_densitymatrix_expecPauliString(rho, coeffs, paulis)
+    e_ref = orc.dm_expecPauliString(ref, coeffs, paulis)
+    assert abs(e - e_ref) <= 2e-6 * max(1.0, abs(e_ref)) * np.linalg.norm(coeffs)
+    k = min(2, N - 1)
+    targs = [int(x) for x in rng.permutation(N)[:k]]
+    out = fsa.distributed_densitymatrix_partialTrace(rho, targs)
+    assert rel(out.download(), orc.dm_partialTrace(ref, targs)) <= 2e-6
+
+
+def unitary_like(rng, d):
+    u, _, vh = np.linalg.svd(rm(rng, d))
+    return u @ vh
+
+
 @pytest.mark.parametrize("N", [1, 3, 5, 8, 11])
 def test_dm_expec_pauli_string(fsa, orc, N):
     rng = np.random.default_rng(400 + N)
