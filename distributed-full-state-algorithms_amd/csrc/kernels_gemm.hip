@@ -11,9 +11,13 @@
 // 16 per thread, all loads in flight at once), each in address order (a thread's low eight
 // local-index bits are its id, so a wave reads consecutive amplitudes where
 // the low bits are not targets).  Four waves split the work into units of 16
-// columns x (16 RBW) rows: the real GEMM [Yr; Yi] = [[Ur, -Ui], [Ui, Ur]]
-// [Xr; Xi] on v_mfma_*_16x16x4, four MFMAs per 16-row block and k-step
-// (Ur Xr, -Ui Xi into Yr; Ui Xr, Ur Xi into Yi).  The A operand comes from the
+// columns x (16 RBW) rows.  The complex product takes three real MFMAs per
+// 16-row block and k-step on v_mfma_*_16x16x4 (Gauss's three-multiplication
+// form): P1 = Ur Xr, P2 = Ui Xi and P3 = (Ur + Ui)(Xr + Xi) accumulate
+// apart, and Yr = P1 - P2, Yi = P3 - P1 - P2 at the end, a quarter fewer
+// matrix operations than the four products of [[Ur, -Ui], [Ui, Ur]] [Xr; Xi]
+// (FSA_GEMM_3M=0 keeps those).  The sums Ur + Ui and Xr + Xi are formed per
+// lane as the operands arrive.  The A operand comes from the
 // matrix packed in MFMA fragment order, one 2 x sizeof(R) load per lane and
 // fragment from L2; the B operand is one LDS read of the staged amplitude.
 //
@@ -64,7 +68,7 @@ __device__ __forceinline__ void gemm_local(const GemmGeom& g, uint32_t x, uint32
         }
 }
 
-template <typename R, int RBW>
+template <typename R, int RBW, bool G3>
 __global__ __launch_bounds__(GEMM_THREADS, 2) void k_matk_gemm(cx<R>* __restrict__ a,
                                                               const GemmFrag<R>* __restrict__ frag, GemmGeom g,
                                                               uint64_t cblkFirst, cx<R>* __restrict__ scratch) {
@@ -98,9 +102,10 @@ __global__ __launch_bounds__(GEMM_THREADS, 2) void k_matk_gemm(cx<R>* __restrict
     const uint32_t ru = wave % g.rowUnits, cu = wave / g.rowUnits;
     const uint32_t rbFirst = (panel * g.rowUnits + ru) * RBW;
     const uint32_t colInBlk = cu * 16u + lr;
-    acc_t yr[RBW], yi[RBW];
+    // G3: yr, yi, ys accumulate P1, P2, P3; otherwise yr, yi are Yr, Yi
+    acc_t yr[RBW], yi[RBW], ys[RBW];
 #pragma unroll
-    for (int r = 0; r < RBW; r++) yr[r] = yi[r] = acc_t{0, 0, 0, 0};
+    for (int r = 0; r < RBW; r++) yr[r] = yi[r] = ys[r] = acc_t{0, 0, 0, 0};
     const uint32_t kPerRb = D / 4;                          // fragments per 16-row block
     const uint32_t nChunks = D >> g.kcBits;
     const uint32_t nks = KC / 4;                            // k-steps per chunk: 4 or more, even
@@ -132,12 +137,22 @@ __global__ __launch_bounds__(GEMM_THREADS, 2) void k_matk_gemm(cx<R>* __restrict
             x = xb[ks * 4u * CBW];
         };
         auto step = [&](const GemmFrag<R>* m, const cx<R>& x) {
+            if constexpr (G3) {
+                const R xsum = x.x + x.y;
 #pragma unroll
-            for (int r = 0; r < RBW; r++) {
-                yr[r] = A::mma(m[r].re, x.x, yr[r]);
-                yi[r] = A::mma(m[r].im, x.x, yi[r]);
-                yr[r] = A::mma(-m[r].im, x.y, yr[r]);
-                yi[r] = A::mma(m[r].re, x.y, yi[r]);
+                for (int r = 0; r < RBW; r++) {
+                    yr[r] = A::mma(m[r].re, x.x, yr[r]);
+                    yi[r] = A::mma(m[r].im, x.y, yi[r]);
+                    ys[r] = A::mma(m[r].re + m[r].im, xsum, ys[r]);
+                }
+            } else {
+#pragma unroll
+                for (int r = 0; r < RBW; r++) {
+                    yr[r] = A::mma(m[r].re, x.x, yr[r]);
+                    yi[r] = A::mma(m[r].im, x.x, yi[r]);
+                    yr[r] = A::mma(-m[r].im, x.y, yr[r]);
+                    yi[r] = A::mma(m[r].re, x.y, yi[r]);
+                }
             }
         };
         load(m0, x0, 0);
@@ -150,6 +165,14 @@ __global__ __launch_bounds__(GEMM_THREADS, 2) void k_matk_gemm(cx<R>* __restrict
         }
     }
 
+    if constexpr (G3) {
+#pragma unroll
+        for (int r = 0; r < RBW; r++) {
+            const acc_t p1 = yr[r], p2 = yi[r];
+            yr[r] = p1 - p2;
+            yi[r] = ys[r] - p1 - p2;
+        }
+    }
     if (g.inplace) {
         // one panel, one chunk: the block is this workgroup's alone
         __syncthreads();
@@ -270,6 +293,12 @@ void gemm_pack_matrix(const std::complex<double>* m, unsigned k, GemmFrag<R>* ou
             }
 }
 
+// FSA_GEMM_3M=0: four real products per complex one instead of three
+static bool gemm_three_mult() {
+    const char* e = std::getenv("FSA_GEMM_3M");
+    return !(e && e[0] == '0');
+}
+
 uint64_t gemm_scratch_elems(const GemmGeom& g, unsigned nq, size_t realBytes) {
     if (g.inplace) return 0;
     const uint64_t perBlk = 1ULL << (g.k + g.cbits);
@@ -286,7 +315,8 @@ template <typename R>
 void launch_matk_gemm(hipStream_t s, cx<R>* a, unsigned nq, const GemmGeom& g, const GemmFrag<R>* dfrag,
                       cx<R>* scratch) {
     const uint64_t numCblk = 1ULL << (nq - g.k - g.cbits);
-    auto kern = k_matk_gemm<R, 4>;        // k >= 6: four 16-row blocks per wave
+    // k >= 6: four 16-row blocks per wave
+    auto kern = gemm_three_mult() ? k_matk_gemm<R, 4, true> : k_matk_gemm<R, 4, false>;
     if (g.inplace) {
         hipLaunchKernelGGL(kern, dim3((unsigned)numCblk), dim3(GEMM_THREADS), 0, s, a, dfrag, g, (uint64_t)0,
                            (cx<R>*)nullptr);

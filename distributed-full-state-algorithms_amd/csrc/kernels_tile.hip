@@ -349,7 +349,8 @@ __global__ __launch_bounds__(TB, (sizeof(R) == 8 && ROWS >= 16) ? 2 : 4) void k_
     uint64_t tile = blockIdx.x;
     if (tile >= numTiles) return;
     // MFMA matrices with an LDS offset are copied once, behind the tile, in the
-    // order mat_lds_index reads them (the left half of the real form)
+    // order mat_lds_index reads them: the left half of the real form, or for
+    // the FP64 three-product form (tile_mfma3_form) U itself as complex entries
     R* lmat = reinterpret_cast<R*>(smem_raw + ((size_t)sizeof(cx<R>) << m));
     if constexpr (VAR == 2) {
         for (unsigned it = 0; it < numItems; it++) {
@@ -357,6 +358,15 @@ __global__ __launch_bounds__(TB, (sizeof(R) == 8 && ROWS >= 16) ? 2 : 4) void k_
             if (op.kind != OP_MATK || op.lmask2 != 1u || op.lmask == MAT_UNPLACED) continue;
             const uint32_t k = op.k, D = 1u << k;
             const R* M = reinterpret_cast<const R*>(mats + op.mat_off);
+            if (tile_mfma3_form<R, TB>(k)) {
+                for (uint32_t e = tid; e < D * D; e += TB) {
+                    const uint32_t r = e >> k, c = e & (D - 1);
+                    R* dst = lmat + op.lmask + 2 * (r * D + (c ^ (r & (D - 1))));
+                    dst[0] = M[(size_t)r * 2 * D + c];
+                    dst[1] = M[(size_t)(r + D) * 2 * D + c];
+                }
+                continue;
+            }
             for (uint32_t e = tid; e < 2 * D * D; e += TB) {
                 const uint32_t r = e >> k, c = e & (D - 1);
                 lmat[op.lmask + r * D + (c ^ (r & (D - 1)))] = M[(size_t)r * 2 * D + c];

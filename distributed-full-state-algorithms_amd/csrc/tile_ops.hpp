@@ -256,6 +256,107 @@ __device__ __forceinline__ void tile_matk_mfma(cx<R>* s, uint32_t T, const TileO
     }
 }
 
+// FP64 gates of 4 or 5 targets in Gauss's three-product form (matrices in LDS,
+// 512-thread workgroups: tile_mfma3_form): a wave task is one 16-row block of U
+// (D / 16 of them) against two 16-group
+// column blocks.  A first sweep over the D / 4 k-steps accumulates P1 = Ur Xr
+// and P2 = Ui Xi, which become Yr = P1 - P2 and -(P1 + P2); a second sweep adds
+// P3 = (Ur + Ui)(Xr + Xi) to the latter, giving Yi.  Three quarters of the real
+// form's matrix operations, the same accumulator registers, and the same LDS
+// operand traffic (each sweep reads every amplitude once per row block of U,
+// as a 16-byte pair).  The tasks per tile are those of the real form (T / 512
+// for every K).  M holds U as complex entries, swizzled like the real form
+// (mat_lds_index), so that one 16-byte LDS read gives ur and ui.
+// The form is fixed at compile time: with the real form of these gates also
+// compiled in, the FP64 kernels exceed 128 VGPRs and spill; the 1024-thread
+// kernels spill with it and keep the real form.
+template <typename R, int TB>
+__host__ __device__ constexpr bool tile_mfma3_form(uint32_t k) {
+    return sizeof(R) == 8 && TB == 512 && (k == 4 || k == 5);
+}
+
+template <int K, int TB, int MAXT>
+__device__ __forceinline__ void tile_matk_mfma3(cx<double>* s, uint32_t T, const TileOp& op,
+                                                const cx<double>* __restrict__ M) {
+    typedef MfmaAcc<double>::T acc_t;
+    static_assert(K == 4 || K == 5, "three-product form: 16 or 32 rows of U");
+    constexpr int D = 1 << K;
+    constexpr int RB = D / 16;                  // 16-row blocks of U
+    constexpr int WAVES = TB / 64;
+    const uint32_t b0 = 1u << op.p[0], b1 = 1u << op.p[1], b2 = 1u << op.p[2], b3 = 1u << op.p[3];
+    const uint32_t b4 = K > 4 ? 1u << op.p[4] : 0u;
+    const uint32_t tm = b0 | b1 | b2 | b3 | b4;
+    const uint32_t groups = T >> K;
+    const uint32_t cb2 = groups / 32;          // pairs of 16-column blocks
+    const uint32_t tasks = RB * cb2;
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    const uint32_t lr = lane & 15u, lk = lane >> 4;
+    acc_t yr[MAXT][2], yi[MAXT][2];
+    uint32_t taskId[MAXT];
+#pragma unroll
+    for (int t = 0; t < MAXT; t++) {
+        taskId[t] = wave + (uint32_t)t * WAVES;
+#pragma unroll
+        for (int c = 0; c < 2; c++) yr[t][c] = yi[t][c] = acc_t{0, 0, 0, 0};
+        if (taskId[t] >= tasks) continue;
+        const uint32_t rb = taskId[t] % RB, cbase = (taskId[t] / RB) * 2;
+        const uint32_t urow = rb * 16 + lr;               // row of U this lane feeds
+        uint32_t gb[2];
+#pragma unroll
+        for (int c = 0; c < 2; c++) gb[c] = ins0_mask32((cbase + (uint32_t)c) * 16 + lr, tm);
+        const cx<double>* mrow = M + urow * (uint32_t)D;
+        const uint32_t sw = urow & (uint32_t)(D - 1);
+        // not unrolled: the MFMAs of the other waves on the SIMD cover the
+        // operand latency, and the loops stay clear of register spills
+#pragma unroll 1
+        for (uint32_t ks = 0; ks < (uint32_t)(D / 4); ks++) {
+            const uint32_t col = ks * 4 + lk;             // column of U = row of X
+            const cx<double> u = mrow[col ^ sw];
+            const uint32_t off = mat_offset<K>(col, b0, b1, b2, b3, b4);
+#pragma unroll
+            for (int c = 0; c < 2; c++) {
+                const cx<double> x = s[gb[c] | off];
+                yr[t][c] = MfmaAcc<double>::step(u.x, x.x, yr[t][c]);
+                yi[t][c] = MfmaAcc<double>::step(u.y, x.y, yi[t][c]);
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < 2; c++) {
+            const acc_t p1 = yr[t][c], p2 = yi[t][c];
+            yr[t][c] = p1 - p2;
+            yi[t][c] = -(p1 + p2);
+        }
+#pragma unroll 1
+        for (uint32_t ks = 0; ks < (uint32_t)(D / 4); ks++) {
+            const uint32_t col = ks * 4 + lk;
+            const cx<double> u = mrow[col ^ sw];
+            const double us = u.x + u.y;
+            const uint32_t off = mat_offset<K>(col, b0, b1, b2, b3, b4);
+#pragma unroll
+            for (int c = 0; c < 2; c++) {
+                const cx<double> x = s[gb[c] | off];
+                yi[t][c] = MfmaAcc<double>::step(us, x.x + x.y, yi[t][c]);
+            }
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int t = 0; t < MAXT; t++) {
+        if (taskId[t] >= tasks) continue;
+        const uint32_t rb = taskId[t] % RB, cbase = (taskId[t] / RB) * 2;
+#pragma unroll
+        for (int c = 0; c < 2; c++) {
+            const uint32_t gbc = ins0_mask32((cbase + (uint32_t)c) * 16 + lr, tm);
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                // accumulator element j of this lane: row 4 j + lane / 16 of U
+                const uint32_t row = rb * 16 + 4u * (uint32_t)j + lk;
+                s[gbc | mat_offset<K>(row, b0, b1, b2, b3, b4)] = mk<double>(yr[t][c][j], yi[t][c][j]);
+            }
+        }
+    }
+}
+
 // Kernel variants: VAR 0 carries the simple LDS ops (and, with SW, register
 // sweeps); VAR 1 adds VALU dense gates and two-qubit depolarising; VAR 2 has
 // MFMA dense gates (matrices in LDS) instead of VALU ones, VAR 3 the same with
@@ -330,6 +431,14 @@ __device__ __forceinline__ void tile_apply(cx<R>* s, uint32_t T, uint64_t base, 
                 static_assert(sizeof(C) == sizeof(R), "MFMA tiles compute in their storage precision");
                 constexpr bool LDSA = VAR == 2;   // VAR 2: every MFMA matrix of the pass is in LDS
                 const R* M = LDSA ? lmat + op.lmask : reinterpret_cast<const R*>(mat);
+                if constexpr (LDSA && tile_mfma3_form<R, TB>(4)) {
+                    if (op.k >= 4) {          // three-product form, complex U in LDS
+                        const cx<double>* Mc = reinterpret_cast<const cx<double>*>(M);
+                        if (op.k == 4) tile_matk_mfma3<4, TB, MAXT>(s, T, op, Mc);
+                        else tile_matk_mfma3<5, TB, MAXT>(s, T, op, Mc);
+                        return;
+                    }
+                }
                 if (op.k == 3) tile_matk_mfma<R, 3, TB, MAXT, LDSA>(s, T, op, M);
                 else if (op.k == 4) tile_matk_mfma<R, 4, TB, MAXT, LDSA>(s, T, op, M);
                 else tile_matk_mfma<R, 5, TB, MAXT, LDSA>(s, T, op, M);

@@ -336,12 +336,13 @@ def test_standalone_dense_gate_between_passes_on_cpu(drive, orc, tmp_path):
     assert out["rel_err"] <= 1e-12 and out["passes"] == 3 and out["tiled"] == 2, out
 
 
-@pytest.mark.parametrize("k,n,eager,fp32", [(6, 12, 1, 0), (6, 13, 0, 1), (7, 12, 1, 1), (8, 12, 0, 0),
-                                             (9, 14, 0, 1)])
-def test_dense_gate_gemm_on_cpu(drive, orc, tmp_path, k, n, eager, fp32):
+@pytest.mark.parametrize("k,n,eager,fp32,g3", [(6, 12, 1, 0, 1), (6, 13, 0, 1, 1), (7, 12, 1, 1, 1),
+                                                (8, 12, 0, 0, 1), (9, 14, 0, 1, 1), (7, 13, 1, 0, 0)])
+def test_dense_gate_gemm_on_cpu(drive, orc, tmp_path, k, n, eager, fp32, g3):
     # kernels_gemm.hip: in place for k <= 8, row panels through the scratch
     # area above (a 1 KiB cap: one column block per batch, so several
-    # batches); targets in shuffled order so that low qubits are among them
+    # batches); targets in shuffled order so that low qubits are among them.
+    # g3: three real products per complex one (FSA_GEMM_3M, the default) or four
     rng = np.random.default_rng(70 + k)
     q = [int(x) for x in rng.permutation(n)[:k]]
     ops = [("manyTargGate", (q, _unitary(rng, 1 << k))), ("oneTargGate", (q[0], _unitary(rng, 2)))]
@@ -349,7 +350,7 @@ def test_dense_gate_gemm_on_cpu(drive, orc, tmp_path, k, n, eager, fp32):
         ops.append(("manyTargGate", (sorted(q)[::-1], _unitary(rng, 1 << k))))
     path = tmp_path / "gemm.bin"
     _circuit_file(path, orc, n, ops, rng)
-    env = {"FSA_GEMM_SCRATCH_KB": "1"}
+    env = {"FSA_GEMM_SCRATCH_KB": "1", "FSA_GEMM_3M": str(g3)}
     if eager:
         env["EMU_EAGER"] = "1"
     if fp32:
