@@ -907,6 +907,18 @@ void execute_impl(State& s, const std::vector<HOp>& ops, const std::vector<Stage
                 used += need;
             }
             matLds[si] = used;
+        } else if (stageMfma) {
+            // VAR 3: each matrix is staged at offset 0 just before its item
+            uint32_t most = 0;
+            for (size_t i = first[si]; i < items.size(); i++) {
+                TileOp& it = items[i];
+                if (it.kind != OP_MATK || it.lmask2 != 1u) continue;
+                it.lmask = 0;
+                most = std::max(most, (uint32_t)(2u * sizeof(R)) << (2 * it.k));
+            }
+            if (most > tile_mat_lds_bytes(sizeof(cx<R>) << m))
+                fail(FSA_ERR_INTERNAL, "an MFMA matrix exceeds the LDS area of a tile pass");
+            matLds[si] = most;
         }
         // kernel variant: 1 for dense gates / two-qubit depolarising, 2 for MFMA
         // dense gates with matrices in LDS, 3 with matrices in HBM; bit 4 marks sweeps
@@ -956,7 +968,7 @@ void execute_impl(State& s, const std::vector<HOp>& ops, const std::vector<Stage
             // one profile name per kernel instantiation family, so the timings
             // line up with per-kernel profiler statistics
             const uint32_t var = (flags[si] >> 2) & 3u;
-            ProfScope prof(var == 3 ? "tile_pass_mfma_hbm" : var == 2 ? "tile_pass_mfma" : var == 1 ? "tile_pass_dense"
+            ProfScope prof(var == 3 ? "tile_pass_mfma_staged" : var == 2 ? "tile_pass_mfma" : var == 1 ? "tile_pass_dense"
                            : (flags[si] & 16u) ? "tile_pass_sweep" : "tile_pass_plain", s.stream);
             launch_tile<R>(s.stream, static_cast<cx<R>*>(s.amps), s.nq, geoms[si], ditems + first[si],
                            (unsigned)count[si], dmicro, dmats, flags[si], matLds[si]);

@@ -348,29 +348,33 @@ __global__ __launch_bounds__(TB, (sizeof(R) == 8 && ROWS >= 16) ? 2 : 4) void k_
     R bx[ROWS], by[ROWS];
     uint64_t tile = blockIdx.x;
     if (tile >= numTiles) return;
-    // MFMA matrices with an LDS offset are copied once, behind the tile, in the
+    // MFMA matrices go to LDS behind the tile, at their item's offset, in the
     // order mat_lds_index reads them: the left half of the real form, or for
-    // the FP64 three-product form (tile_mfma3_form) U itself as complex entries
+    // the FP64 three-product form (tile_mfma3_form) U itself as complex
+    // entries.  VAR 2 copies every matrix of the pass once; VAR 3 (matrices
+    // beyond the LDS area) copies each one just before its item, at offset 0.
     R* lmat = reinterpret_cast<R*>(smem_raw + ((size_t)sizeof(cx<R>) << m));
+    auto stage_matrix = [&](const TileOp& op) {
+        const uint32_t k = op.k, D = 1u << k;
+        const R* M = reinterpret_cast<const R*>(mats + op.mat_off);
+        if (tile_mfma3_form<R, TB>(k)) {
+            for (uint32_t e = tid; e < D * D; e += TB) {
+                const uint32_t r = e >> k, c = e & (D - 1);
+                R* dst = lmat + op.lmask + 2 * (r * D + (c ^ (r & (D - 1))));
+                dst[0] = M[(size_t)r * 2 * D + c];
+                dst[1] = M[(size_t)(r + D) * 2 * D + c];
+            }
+            return;
+        }
+        for (uint32_t e = tid; e < 2 * D * D; e += TB) {
+            const uint32_t r = e >> k, c = e & (D - 1);
+            lmat[op.lmask + r * D + (c ^ (r & (D - 1)))] = M[(size_t)r * 2 * D + c];
+        }
+    };
     if constexpr (VAR == 2) {
         for (unsigned it = 0; it < numItems; it++) {
             const TileOp& op = items[it];
-            if (op.kind != OP_MATK || op.lmask2 != 1u || op.lmask == MAT_UNPLACED) continue;
-            const uint32_t k = op.k, D = 1u << k;
-            const R* M = reinterpret_cast<const R*>(mats + op.mat_off);
-            if (tile_mfma3_form<R, TB>(k)) {
-                for (uint32_t e = tid; e < D * D; e += TB) {
-                    const uint32_t r = e >> k, c = e & (D - 1);
-                    R* dst = lmat + op.lmask + 2 * (r * D + (c ^ (r & (D - 1))));
-                    dst[0] = M[(size_t)r * 2 * D + c];
-                    dst[1] = M[(size_t)(r + D) * 2 * D + c];
-                }
-                continue;
-            }
-            for (uint32_t e = tid; e < 2 * D * D; e += TB) {
-                const uint32_t r = e >> k, c = e & (D - 1);
-                lmat[op.lmask + r * D + (c ^ (r & (D - 1)))] = M[(size_t)r * 2 * D + c];
-            }
+            if (op.kind == OP_MATK && op.lmask2 == 1u && op.lmask != MAT_UNPLACED) stage_matrix(op);
         }
         __syncthreads();
     }
@@ -391,6 +395,13 @@ __global__ __launch_bounds__(TB, (sizeof(R) == 8 && ROWS >= 16) ? 2 : 4) void k_
                     sweep_lds<R, TB, W>(s, T, op, micro, base, mats);
                     __syncthreads();
                     continue;
+                }
+            }
+            if constexpr (VAR == 3) {
+                // the previous item's barrier: every wave is done with the area
+                if (op.kind == OP_MATK && op.lmask2 == 1u) {
+                    stage_matrix(op);
+                    __syncthreads();
                 }
             }
             tile_apply<R, TB, VAR, (ROWS >= 16 ? ROWS / 8 : 1)>(s, T, base, op, mats, lmat);
@@ -476,7 +487,7 @@ void launch_tile(hipStream_t s, cx<R>* a, unsigned nq, const TileGeom& geom, con
     uint64_t numTiles = 1ULL << (nq - geom.m);
     const uint32_t var = (flags >> 2) & 3u;
     const size_t tileBytes = sizeof(cx<R>) << geom.m;
-    if (matLds > (var == 2 ? tile_mat_lds_bytes(tileBytes) : 0u))
+    if (matLds > (var >= 2 ? tile_mat_lds_bytes(tileBytes) : 0u))
         fail(FSA_ERR_INTERNAL, "tile pass asks for more LDS for matrices than its variant has");
     const size_t lds = tileBytes + matLds;
     // 16 waves per CU either way: two workgroups of 512 threads for tiles of up

@@ -74,7 +74,7 @@ void launch_scale(hipStream_t s, cx<R>* a, uint64_t num, R f);
 // matrices in HBM), bit 4: the pass has sweeps, bits 8-15: the number of
 // trailing OP_DIAG4 items (0 or 1) applied in registers as each tile is
 // written back.
-// matLds: bytes of LDS after the tile for MFMA matrices (variant 2 only).
+// matLds: bytes of LDS after the tile for MFMA matrices (variants 2 and 3).
 template <typename R>
 void launch_tile(hipStream_t s, cx<R>* a, unsigned nq, const TileGeom& geom, const TileOp* ditems,
                  unsigned numItems, const TileOp* dmicro, const cx<R>* dmats, uint32_t flags, uint32_t matLds = 0);

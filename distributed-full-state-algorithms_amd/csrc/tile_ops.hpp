@@ -164,12 +164,10 @@ template <> struct MfmaAcc<float> {
 // so each of the TB / 64 waves takes MAXT = T / (8 TB) of them: 1 for 4096
 // amplitudes at TB = 512, 2 for 8192.
 //
-// The A operand (the real matrix [[Ur, -Ui], [Ui, Ur]]) is read from LDS
-// (LDSA: M holds its left half, the 2D x D reals [Ur; Ui], stored by
-// mat_lds_index) or from HBM (M is the row-major 2D x 2D real form).  LDS
-// reads keep the item loop free of vector-memory loads, whose waits would
-// also wait for the next tile's prefetch (loads complete in order); HBM reads
-// serve passes whose matrices do not fit in the LDS area.
+// The A operand (the real matrix [[Ur, -Ui], [Ui, Ur]]) is read from LDS: M
+// holds its left half, the 2D x D reals [Ur; Ui], stored by mat_lds_index.
+// LDS reads keep the MFMA loops free of vector-memory loads, whose waits
+// would also wait for the next tile's prefetch (loads complete in order).
 template <int D>
 __device__ __forceinline__ uint32_t mat_lds_index(uint32_t r, uint32_t c) {
     // columns XOR-swizzled by the row: the 16 rows a wave reads at one column
@@ -177,7 +175,7 @@ __device__ __forceinline__ uint32_t mat_lds_index(uint32_t r, uint32_t c) {
     return r * (uint32_t)D + (c ^ (r & (uint32_t)(D - 1)));
 }
 
-template <typename R, int K, int TB, int MAXT, bool LDSA>
+template <typename R, int K, int TB, int MAXT>
 __device__ __forceinline__ void tile_matk_mfma(cx<R>* s, uint32_t T, const TileOp& op, const R* __restrict__ M) {
     typedef typename MfmaAcc<R>::T acc_t;
     constexpr int D = 1 << K, D2 = 2 * D;
@@ -215,26 +213,16 @@ __device__ __forceinline__ void tile_matk_mfma(cx<R>* s, uint32_t T, const TileO
             for (int c = 0; c < 4; c++) acc[t][c] = MfmaAcc<R>::step(a, sr[2 * (gb[c] | off) + half], acc[t][c]);
         };
         // not unrolled: four independent 64-cycle FP64 MFMAs per k-step cover
-        // the operand latency, and the loop stays clear of register spills
-        if constexpr (LDSA) {
-            // columns C < D: A[row][C] = left[row][C]; columns C >= D:
-            // A[row][C] = -left[row + D][C - D] (row < D, the -Ui block) or
-            // left[row - D][C - D] (row >= D, the Ur block)
-            const uint32_t r2 = (arow + D) & (D2 - 1);
-            const R sgn = arow < (uint32_t)D ? (R)-1 : (R)1;
+        // the operand latency, and the loop stays clear of register spills.
+        // Columns C < D: A[row][C] = left[row][C]; columns C >= D:
+        // A[row][C] = -left[row + D][C - D] (row < D, the -Ui block) or
+        // left[row - D][C - D] (row >= D, the Ur block)
+        const uint32_t r2 = (arow + D) & (D2 - 1);
+        const R sgn = arow < (uint32_t)D ? (R)-1 : (R)1;
 #pragma unroll 1
-            for (int ks = 0; ks < D / 4; ks++) kstep(ks, M[mat_lds_index<D>(arow, (uint32_t)ks * 4 + lk)]);
+        for (int ks = 0; ks < D / 4; ks++) kstep(ks, M[mat_lds_index<D>(arow, (uint32_t)ks * 4 + lk)]);
 #pragma unroll 1
-            for (int ks = D / 4; ks < D2 / 4; ks++)
-                kstep(ks, sgn * M[mat_lds_index<D>(r2, (uint32_t)ks * 4 + lk - D)]);
-        } else {
-            // HBM operands: FP64 unrolled so the loads of two k-steps are in
-            // flight (FP32 tiles hold twice the accumulators: no registers left)
-            const R* ar = M + (size_t)arow * D2;
-            constexpr int HU = sizeof(R) == 8 ? 2 : 1;
-#pragma unroll HU
-            for (int ks = 0; ks < D2 / 4; ks++) kstep(ks, ar[ks * 4 + lk]);
-        }
+        for (int ks = D / 4; ks < D2 / 4; ks++) kstep(ks, sgn * M[mat_lds_index<D>(r2, (uint32_t)ks * 4 + lk - D)]);
     }
     __syncthreads();
 #pragma unroll
@@ -359,8 +347,9 @@ __device__ __forceinline__ void tile_matk_mfma3(cx<double>* s, uint32_t T, const
 
 // Kernel variants: VAR 0 carries the simple LDS ops (and, with SW, register
 // sweeps); VAR 1 adds VALU dense gates and two-qubit depolarising; VAR 2 has
-// MFMA dense gates (matrices in LDS) instead of VALU ones, VAR 3 the same with
-// the matrices read from HBM.  Each instantiation stays within 128 VGPRs so
+// MFMA dense gates (matrices in LDS for the whole pass) instead of VALU ones,
+// VAR 3 the same with each matrix staged in LDS just before its item (passes
+// whose matrices exceed the LDS area).  Each instantiation stays within 128 VGPRs so
 // two 64 KiB workgroups share a CU.
 // lmat: the pass's LDS matrix area (VAR 2: every MFMA item has an offset in it)
 template <typename R, int TB, int VAR, int MAXT = 1, typename C = R>
@@ -429,9 +418,9 @@ __device__ __forceinline__ void tile_apply(cx<R>* s, uint32_t T, uint64_t base, 
         if (op.lmask2 == 1u) {        // host chose the MFMA form (full tile, k = 3, 4 or 5)
             if constexpr (VAR >= 2) {
                 static_assert(sizeof(C) == sizeof(R), "MFMA tiles compute in their storage precision");
-                constexpr bool LDSA = VAR == 2;   // VAR 2: every MFMA matrix of the pass is in LDS
-                const R* M = LDSA ? lmat + op.lmask : reinterpret_cast<const R*>(mat);
-                if constexpr (LDSA && tile_mfma3_form<R, TB>(4)) {
+                // the matrix is in LDS: staged for the whole pass (VAR 2) or for this item (VAR 3)
+                const R* M = lmat + op.lmask;
+                if constexpr (tile_mfma3_form<R, TB>(4)) {
                     if (op.k >= 4) {          // three-product form, complex U in LDS
                         const cx<double>* Mc = reinterpret_cast<const cx<double>*>(M);
                         if (op.k == 4) tile_matk_mfma3<4, TB, MAXT>(s, T, op, Mc);
@@ -439,9 +428,9 @@ __device__ __forceinline__ void tile_apply(cx<R>* s, uint32_t T, uint64_t base, 
                         return;
                     }
                 }
-                if (op.k == 3) tile_matk_mfma<R, 3, TB, MAXT, LDSA>(s, T, op, M);
-                else if (op.k == 4) tile_matk_mfma<R, 4, TB, MAXT, LDSA>(s, T, op, M);
-                else tile_matk_mfma<R, 5, TB, MAXT, LDSA>(s, T, op, M);
+                if (op.k == 3) tile_matk_mfma<R, 3, TB, MAXT>(s, T, op, M);
+                else if (op.k == 4) tile_matk_mfma<R, 4, TB, MAXT>(s, T, op, M);
+                else tile_matk_mfma<R, 5, TB, MAXT>(s, T, op, M);
             }
             return;
         }
