@@ -74,8 +74,21 @@ static bool env_flag(const char* name, bool dflt) {
     return v[0] != '0';
 }
 
-SchedParams sched_params(unsigned nq, int precision) {
-    const unsigned mmax = precision == FSA_FP64 ? (unsigned)tile_bits_max<double>() : (unsigned)tile_bits_max<float>();
+bool has_mfma_gates(const std::vector<HOp>& ops) {
+    for (const HOp& op : ops)
+        if (op.kind == OP_MATK && op.q.size() >= 3 && op.q.size() <= 5) return true;
+    return false;
+}
+
+SchedParams sched_params(unsigned nq, int precision, bool mfmaGates) {
+    // FP32 tiles hold 2^13 amplitudes (64 KiB, two workgroups per CU) where
+    // dense gates of 3-5 targets need them: their MFMA form works on full
+    // tiles.  Without such gates, 2^12 (32 KiB) lets four workgroups share a
+    // CU, which hides more of each item's latency: the 24-qubit gadget
+    // circuit's forward passes take 6.8 instead of 7.3 ms.
+    const unsigned mmax = precision == FSA_FP64 ? (unsigned)tile_bits_max<double>()
+                          : mfmaGates            ? (unsigned)tile_bits_max<float>()
+                                                 : (unsigned)tile_bits_max<float>() - 1u;
     const unsigned mlim = precision == FSA_FP64 ? (unsigned)tile_bits_limit<double>() : (unsigned)tile_bits_limit<float>();
     unsigned m = std::min(env_u("FSA_TILE_BITS", mmax), mlim);
     // Qubits 0..low-1 are always tile qubits, so every HBM access of a pass is
@@ -271,7 +284,7 @@ std::vector<HOp> relabel_swaps(const std::vector<HOp>& ops, unsigned nq) {
 Plan make_plan(const std::vector<HOp>& ops, unsigned nq, int precision) {
     Plan p;
     p.ops = relabel_swaps(ops, nq);
-    p.stages = schedule(p.ops, nq, sched_params(nq, precision));
+    p.stages = schedule(p.ops, nq, sched_params(nq, precision, has_mfma_gates(p.ops)));
     return p;
 }
 
@@ -1001,7 +1014,7 @@ void execute_now(State& s, const std::vector<HOp>& ops) {
         execute(s, ops, {st});
         return;
     }
-    execute(s, ops, schedule(ops, s.nq, sched_params(s.nq, s.prec)));
+    execute(s, ops, schedule(ops, s.nq, sched_params(s.nq, s.prec, has_mfma_gates(ops))));
 }
 
 void flush(State& s) {

@@ -63,7 +63,10 @@ struct SchedParams {
     unsigned m;      // tile bits
     unsigned low;    // low qubits always in the tile (coalescing)
 };
-SchedParams sched_params(unsigned nq, int precision);
+// mfmaGates: the ops hold dense gates of 3-5 targets (FP32 then keeps the
+// 2^13-amplitude tiles their MFMA form needs)
+SchedParams sched_params(unsigned nq, int precision, bool mfmaGates = true);
+bool has_mfma_gates(const std::vector<HOp>& ops);
 std::vector<Stage> schedule(const std::vector<HOp>& ops, unsigned nq, SchedParams sp);
 
 // A fused plan: swap gates become relabelings of the later ops' qubits, the

@@ -13,6 +13,8 @@
 #include "tile_ops.hpp"
 
 #include <algorithm>
+#include <cstdlib>
+#include <map>
 #include <mutex>
 #include <unordered_map>
 
@@ -463,6 +465,25 @@ __global__ __launch_bounds__(TB, (sizeof(R) == 8 && ROWS >= 16) ? 2 : 4) void k_
 #undef FSA_PREFETCH
 }
 
+// Resident workgroups per CU for a tile kernel at this LDS size (cached; at most
+// 4).  FSA_TILE_OCC=0 keeps the fixed count `dflt` (two 512-thread workgroups,
+// or one of 1024).
+unsigned tile_workgroups_per_cu(const void* kern, unsigned tb, size_t lds, unsigned dflt) {
+    const char* e = std::getenv("FSA_TILE_OCC");
+    if (e && e[0] == '0') return dflt;
+    static std::mutex mu;
+    static std::map<std::pair<const void*, size_t>, unsigned> cache;
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = cache.find({kern, lds});
+    if (it != cache.end()) return it->second;
+    int occ = 0;
+    unsigned n = dflt;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, (int)tb, lds) == hipSuccess && occ > 0)
+        n = std::min(4u, (unsigned)occ);
+    cache[{kern, lds}] = n;
+    return n;
+}
+
 // kernel of a pass: TB threads, ROWS = tile / TB amplitudes per thread
 template <typename R, int TB, int RW>
 auto pick_tile(uint32_t var, bool sw) {
@@ -490,11 +511,10 @@ void launch_tile(hipStream_t s, cx<R>* a, unsigned nq, const TileGeom& geom, con
     if (matLds > (var >= 2 ? tile_mat_lds_bytes(tileBytes) : 0u))
         fail(FSA_ERR_INTERNAL, "tile pass asks for more LDS for matrices than its variant has");
     const size_t lds = tileBytes + matLds;
-    // 16 waves per CU either way: two workgroups of 512 threads for tiles of up
-    // to 64 KiB, one of 1024 threads for a 128 KiB tile
+    // 512-thread workgroups for tiles of up to 64 KiB (two per CU at 64 KiB), one
+    // of 1024 threads for a 128 KiB tile
     const bool big = tileBytes > 64 * 1024;
     const uint32_t tb = big ? 1024u : 512u;
-    const uint64_t grid = std::min<uint64_t>(numTiles, (uint64_t)numCU * (big ? 1u : 2u));
     const bool sw = (flags & 16u) != 0;
     auto* kern = k_tile<R, 512, 1, sweep_bits<R>(), 0, false>;
     if (big) {
@@ -526,6 +546,11 @@ void launch_tile(hipStream_t s, cx<R>* a, unsigned nq, const TileGeom& geom, con
             g = lds;
         }
     }
+    // The persistent grid is as many workgroups as fit on the GPU at once: what
+    // the kernel's registers and this pass's LDS allow per CU (smaller tiles
+    // leave room for more), so every workgroup is resident from the start.
+    const unsigned perCU = tile_workgroups_per_cu(reinterpret_cast<const void*>(kern), tb, lds, big ? 1u : 2u);
+    const uint64_t grid = std::min<uint64_t>(numTiles, (uint64_t)numCU * perCU);
     hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(tb), lds, s, a, geom, ditems, numItems, dmicro, dmats,
                        numTiles, flags);
 }
