@@ -406,7 +406,10 @@ __global__ __launch_bounds__(TB, (sizeof(R) == 8 && ROWS >= 16) ? 2 : 4) void k_
                     __syncthreads();
                 }
             }
-            tile_apply<R, TB, VAR, (ROWS >= 16 ? ROWS / 8 : 1)>(s, T, base, op, mats, lmat);
+            // batched simple items in the kernel with registers to spare for them
+            // (the plain 512-thread one)
+            tile_apply<R, TB, VAR, (ROWS >= 16 ? ROWS / 8 : 1), R, (VAR == 0 && TB == 512 && !SW ? ROWS : 0)>(
+                s, T, base, op, mats, lmat);
             __syncthreads();
         }
         // The write-back of this tile and the loads of the tile after the next

@@ -352,7 +352,83 @@ __device__ __forceinline__ void tile_matk_mfma3(cx<double>* s, uint32_t T, const
 // whose matrices exceed the LDS area).  Each instantiation stays within 128 VGPRs so
 // two 64 KiB workgroups share a CU.
 // lmat: the pass's LDS matrix area (VAR 2: every MFMA item has an offset in it)
-template <typename R, int TB, int VAR, int MAXT = 1, typename C = R>
+// Simple items over n pairs (i0, i1) = idx(j) or n single amplitudes l = j,
+// j = tid, tid + TB, ...  With KB pairs (KB amplitudes) per thread known at
+// compile time and n = KB * TB, the LDS loads of up to 4 FP64 pairs (8
+// amplitudes) are issued before their updates and stores, so the item pays the
+// LDS latency once per group instead of once per pair; the slots of different
+// j are distinct, so the early loads read nothing the stores change.  FP32
+// groups are one pair (two singles): that keeps the FP32 kernel within 64
+// VGPRs, four workgroups per CU.  Otherwise (KB < 2, partial tiles) one pair
+// at a time.
+template <typename R, int TB, int KB, typename C, typename Idx, typename Upd>
+__device__ __forceinline__ void tile_pairs(cx<R>* s, uint32_t n, Idx idx, Upd upd) {
+    const uint32_t tid = threadIdx.x;
+    if constexpr (KB >= 2) {
+        constexpr int GM = sizeof(C) == 8 ? 4 : 1;
+        constexpr int G = KB < GM ? KB : GM;
+        if (n == (uint32_t)KB * TB) {
+#pragma unroll
+            for (int k0 = 0; k0 < KB; k0 += G) {
+                uint32_t i0[G], i1[G];
+                cx<C> v0[G], v1[G];
+#pragma unroll
+                for (int k = 0; k < G; k++) {
+                    idx(tid + (uint32_t)(k0 + k) * TB, i0[k], i1[k]);
+                    v0[k] = up<C>(s[i0[k]]);
+                    v1[k] = up<C>(s[i1[k]]);
+                }
+#pragma unroll
+                for (int k = 0; k < G; k++) {
+                    upd(i0[k], i1[k], v0[k], v1[k]);
+                    s[i0[k]] = dn<R>(v0[k]);
+                    s[i1[k]] = dn<R>(v1[k]);
+                }
+            }
+            return;
+        }
+    }
+    for (uint32_t j = tid; j < n; j += TB) {
+        uint32_t i0, i1;
+        idx(j, i0, i1);
+        cx<C> v0 = up<C>(s[i0]), v1 = up<C>(s[i1]);
+        upd(i0, i1, v0, v1);
+        s[i0] = dn<R>(v0);
+        s[i1] = dn<R>(v1);
+    }
+}
+
+template <typename R, int TB, int KS, typename C, typename Upd>
+__device__ __forceinline__ void tile_singles(cx<R>* s, uint32_t T, Upd upd) {
+    const uint32_t tid = threadIdx.x;
+    if constexpr (KS >= 2) {
+        constexpr int GM = sizeof(C) == 8 ? 8 : 2;
+        constexpr int G = KS < GM ? KS : GM;
+        if (T == (uint32_t)KS * TB) {
+#pragma unroll
+            for (int k0 = 0; k0 < KS; k0 += G) {
+                cx<C> v[G];
+#pragma unroll
+                for (int k = 0; k < G; k++) v[k] = up<C>(s[tid + (uint32_t)(k0 + k) * TB]);
+#pragma unroll
+                for (int k = 0; k < G; k++) {
+                    upd(tid + (uint32_t)(k0 + k) * TB, v[k]);
+                    s[tid + (uint32_t)(k0 + k) * TB] = dn<R>(v[k]);
+                }
+            }
+            return;
+        }
+    }
+    for (uint32_t l = tid; l < T; l += TB) {
+        cx<C> v = up<C>(s[l]);
+        upd(l, v);
+        s[l] = dn<R>(v);
+    }
+}
+
+// ROWS: amplitudes per thread of a full tile for the batched simple items
+// (0: one pair at a time)
+template <typename R, int TB, int VAR, int MAXT = 1, typename C = R, int ROWS = 0>
 __device__ __forceinline__ void tile_apply(cx<R>* s, uint32_t T, uint64_t base, const TileOp& op,
                            const cx<C>* __restrict__ mats, const R* lmat = nullptr) {
     const uint32_t tid = threadIdx.x;
@@ -363,6 +439,17 @@ __device__ __forceinline__ void tile_apply(cx<R>* s, uint32_t T, uint64_t base, 
         const uint32_t lc = op.lmask;
         const uint32_t nIts = T >> (1 + __builtin_popcount(lc));
         const cx<C> g00 = ldc<C>(op.f, 0), g01 = ldc<C>(op.f, 1), g10 = ldc<C>(op.f, 2), g11 = ldc<C>(op.f, 3);
+        if (lc == 0) {
+            const unsigned t = op.p[0];
+            tile_pairs<R, TB, ROWS / 2, C>(
+                s, nIts, [&](uint32_t j, uint32_t& i0, uint32_t& i1) { i0 = ins0_32(j, t); i1 = i0 | tb; },
+                [&](uint32_t, uint32_t, cx<C>& v0, cx<C>& v1) {
+                    const cx<C> a = v0;
+                    v0 = cfma(cmul(g00, a), g01, v1);
+                    v1 = cfma(cmul(g10, a), g11, v1);
+                });
+            return;
+        }
         for (uint32_t j = tid; j < nIts; j += TB) {
             uint32_t i0 = ins0_mask32(j, lc | tb) | lc;
             cx<C> v0 = up<C>(s[i0]), v1 = up<C>(s[i0 | tb]);
@@ -387,29 +474,29 @@ __device__ __forceinline__ void tile_apply(cx<R>* s, uint32_t T, uint64_t base, 
         const uint32_t bp = par64(base & op.gmask);
         const uint32_t lxy = op.lmask, lyz = op.lmask2;
         if (lxy == 0) {
-            for (uint32_t l = tid; l < T; l += TB) {
-                cx<C> v = up<C>(s[l]);
-                cx<C> f = (bp ^ par32(l & lyz)) ? fo : fe;
-                s[l] = dn<R>(cfma(cmul(th, v), f, v));
-            }
+            tile_singles<R, TB, ROWS, C>(s, T, [&](uint32_t l, cx<C>& v) {
+                const cx<C> f = (bp ^ par32(l & lyz)) ? fo : fe;
+                v = cfma(cmul(th, v), f, v);
+            });
             return;
         }
         const unsigned hb = 31u - (unsigned)__builtin_clz(lxy);
-        for (uint32_t j = tid; j < (T >> 1); j += TB) {
-            uint32_t i0 = ins0_32(j, hb), i1 = i0 ^ lxy;
-            cx<C> v0 = up<C>(s[i0]), v1 = up<C>(s[i1]);
-            cx<C> f0 = (bp ^ par32(i0 & lyz)) ? fo : fe;
-            cx<C> f1 = (bp ^ par32(i1 & lyz)) ? fo : fe;
-            s[i0] = dn<R>(cfma(cmul(th, v0), f1, v1));
-            s[i1] = dn<R>(cfma(cmul(th, v1), f0, v0));
-        }
+        tile_pairs<R, TB, ROWS / 2, C>(
+            s, T >> 1, [&](uint32_t j, uint32_t& i0, uint32_t& i1) { i0 = ins0_32(j, hb); i1 = i0 ^ lxy; },
+            [&](uint32_t i0, uint32_t i1, cx<C>& v0, cx<C>& v1) {
+                const cx<C> f0 = (bp ^ par32(i0 & lyz)) ? fo : fe;
+                const cx<C> f1 = (bp ^ par32(i1 & lyz)) ? fo : fe;
+                const cx<C> a = v0;
+                v0 = cfma(cmul(th, a), f1, v1);
+                v1 = cfma(cmul(th, v1), f0, a);
+            });
         return;
     }
     case OP_PHASE: {
         const cx<C> f0 = ldc<C>(op.f, 0), f1 = ldc<C>(op.f, 1);
         const uint32_t bp = par64(base & op.gmask);
-        for (uint32_t l = tid; l < T; l += TB)
-            s[l] = dn<R>(cmul(up<C>(s[l]), (bp ^ par32(l & op.lmask)) ? f1 : f0));
+        const uint32_t lm = op.lmask;
+        tile_singles<R, TB, ROWS, C>(s, T, [&](uint32_t l, cx<C>& v) { v = cmul(v, (bp ^ par32(l & lm)) ? f1 : f0); });
         return;
     }
     case OP_MATK: {
