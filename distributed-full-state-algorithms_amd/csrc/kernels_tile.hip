@@ -406,10 +406,12 @@ __global__ __launch_bounds__(TB, (sizeof(R) == 8 && ROWS >= 16) ? 2 : 4) void k_
                     __syncthreads();
                 }
             }
-            // batched simple items in the kernel with registers to spare for them
-            // (the plain 512-thread one)
-            tile_apply<R, TB, VAR, (ROWS >= 16 ? ROWS / 8 : 1), R, (VAR == 0 && TB == 512 && !SW ? ROWS : 0)>(
-                s, T, base, op, mats, lmat);
+            // batched simple items, as many pairs per batch as the kernel has
+            // registers for (none in the MFMA kernels)
+            constexpr int GP = (TB != 512 || SW || VAR >= 2) ? 0
+                             : sizeof(R) == 4 ? (VAR == 0 ? 1 : 0)
+                             : VAR == 0 ? 4 : 2;
+            tile_apply<R, TB, VAR, (ROWS >= 16 ? ROWS / 8 : 1), R, (GP ? ROWS : 0), GP>(s, T, base, op, mats, lmat);
             __syncthreads();
         }
         // The write-back of this tile and the loads of the tile after the next

@@ -354,18 +354,18 @@ __device__ __forceinline__ void tile_matk_mfma3(cx<double>* s, uint32_t T, const
 // lmat: the pass's LDS matrix area (VAR 2: every MFMA item has an offset in it)
 // Simple items over n pairs (i0, i1) = idx(j) or n single amplitudes l = j,
 // j = tid, tid + TB, ...  With KB pairs (KB amplitudes) per thread known at
-// compile time and n = KB * TB, the LDS loads of up to 4 FP64 pairs (8
+// compile time and n = KB * TB, the LDS loads of up to GM pairs (2 GM
 // amplitudes) are issued before their updates and stores, so the item pays the
-// LDS latency once per group instead of once per pair; the slots of different
-// j are distinct, so the early loads read nothing the stores change.  FP32
-// groups are one pair (two singles): that keeps the FP32 kernel within 64
-// VGPRs, four workgroups per CU.  Otherwise (KB < 2, partial tiles) one pair
-// at a time.
-template <typename R, int TB, int KB, typename C, typename Idx, typename Upd>
+// LDS latency once per batch instead of once per pair; the slots of different
+// j are distinct, so the early loads read nothing the stores change.  GM comes
+// from the kernel, as many as its registers allow: 4 in the plain FP64 kernel,
+// 2 in the dense FP64 one, 1 in FP32 (which keeps the 32 KiB-tile kernel
+// within 64 VGPRs, four workgroups per CU).  Otherwise (KB < 2, partial tiles)
+// one pair at a time.
+template <typename R, int TB, int KB, int GM, typename C, typename Idx, typename Upd>
 __device__ __forceinline__ void tile_pairs(cx<R>* s, uint32_t n, Idx idx, Upd upd) {
     const uint32_t tid = threadIdx.x;
-    if constexpr (KB >= 2) {
-        constexpr int GM = sizeof(C) == 8 ? 4 : 1;
+    if constexpr (KB >= 2 && GM >= 1) {
         constexpr int G = KB < GM ? KB : GM;
         if (n == (uint32_t)KB * TB) {
 #pragma unroll
@@ -398,11 +398,10 @@ __device__ __forceinline__ void tile_pairs(cx<R>* s, uint32_t n, Idx idx, Upd up
     }
 }
 
-template <typename R, int TB, int KS, typename C, typename Upd>
+template <typename R, int TB, int KS, int GM, typename C, typename Upd>
 __device__ __forceinline__ void tile_singles(cx<R>* s, uint32_t T, Upd upd) {
     const uint32_t tid = threadIdx.x;
-    if constexpr (KS >= 2) {
-        constexpr int GM = sizeof(C) == 8 ? 8 : 2;
+    if constexpr (KS >= 2 && GM >= 1) {
         constexpr int G = KS < GM ? KS : GM;
         if (T == (uint32_t)KS * TB) {
 #pragma unroll
@@ -426,12 +425,46 @@ __device__ __forceinline__ void tile_singles(cx<R>* s, uint32_t T, Upd upd) {
     }
 }
 
-// ROWS: amplitudes per thread of a full tile for the batched simple items
-// (0: one pair at a time)
-template <typename R, int TB, int VAR, int MAXT = 1, typename C = R, int ROWS = 0>
+// Quartets (i, i | b0, i | b1, i | b0 | b1) of two channel bits, GM at a time
+// as tile_pairs does for pairs (n = KB * TB quartets).
+template <typename R, int TB, int KB, int GM, typename C, typename Upd>
+__device__ __forceinline__ void tile_quads(cx<R>* s, uint32_t n, uint32_t b0, uint32_t b1, Upd upd) {
+    const uint32_t tid = threadIdx.x;
+    static_assert(KB >= 2 && GM >= 1, "batched quartets only");
+    constexpr int G = KB < GM ? KB : GM;
+#pragma unroll
+    for (int k0 = 0; k0 < KB; k0 += G) {
+        uint32_t i[G];
+        cx<C> v[G][4];
+#pragma unroll
+        for (int k = 0; k < G; k++) {
+            i[k] = ins0_mask32(tid + (uint32_t)(k0 + k) * TB, b0 | b1);
+            v[k][0] = up<C>(s[i[k]]);
+            v[k][1] = up<C>(s[i[k] | b0]);
+            v[k][2] = up<C>(s[i[k] | b1]);
+            v[k][3] = up<C>(s[i[k] | b0 | b1]);
+        }
+#pragma unroll
+        for (int k = 0; k < G; k++) {
+            upd(v[k]);
+            s[i[k]] = dn<R>(v[k][0]);
+            s[i[k] | b0] = dn<R>(v[k][1]);
+            s[i[k] | b1] = dn<R>(v[k][2]);
+            s[i[k] | b0 | b1] = dn<R>(v[k][3]);
+        }
+    }
+    (void)n;
+}
+
+// ROWS: amplitudes per thread of a full tile for the batched simple items; GP:
+// pairs per batch (singles 2 GP, density-matrix quartets GP / 2 in FP64; 0:
+// one at a time)
+template <typename R, int TB, int VAR, int MAXT = 1, typename C = R, int ROWS = 0, int GP = 0>
 __device__ __forceinline__ void tile_apply(cx<R>* s, uint32_t T, uint64_t base, const TileOp& op,
                            const cx<C>* __restrict__ mats, const R* lmat = nullptr) {
     const uint32_t tid = threadIdx.x;
+    // the density-matrix channels batch in FP64 only (FP32: registers)
+    constexpr int GPC = sizeof(C) == 8 ? GP : 0;
     switch (op.kind) {
     case OP_MAT1: {
         if ((base & op.gmask) != op.gmask) return;           // an outside control is 0
@@ -441,7 +474,7 @@ __device__ __forceinline__ void tile_apply(cx<R>* s, uint32_t T, uint64_t base, 
         const cx<C> g00 = ldc<C>(op.f, 0), g01 = ldc<C>(op.f, 1), g10 = ldc<C>(op.f, 2), g11 = ldc<C>(op.f, 3);
         if (lc == 0) {
             const unsigned t = op.p[0];
-            tile_pairs<R, TB, ROWS / 2, C>(
+            tile_pairs<R, TB, ROWS / 2, GP, C>(
                 s, nIts, [&](uint32_t j, uint32_t& i0, uint32_t& i1) { i0 = ins0_32(j, t); i1 = i0 | tb; },
                 [&](uint32_t, uint32_t, cx<C>& v0, cx<C>& v1) {
                     const cx<C> a = v0;
@@ -474,14 +507,14 @@ __device__ __forceinline__ void tile_apply(cx<R>* s, uint32_t T, uint64_t base, 
         const uint32_t bp = par64(base & op.gmask);
         const uint32_t lxy = op.lmask, lyz = op.lmask2;
         if (lxy == 0) {
-            tile_singles<R, TB, ROWS, C>(s, T, [&](uint32_t l, cx<C>& v) {
+            tile_singles<R, TB, ROWS, 2 * GP, C>(s, T, [&](uint32_t l, cx<C>& v) {
                 const cx<C> f = (bp ^ par32(l & lyz)) ? fo : fe;
                 v = cfma(cmul(th, v), f, v);
             });
             return;
         }
         const unsigned hb = 31u - (unsigned)__builtin_clz(lxy);
-        tile_pairs<R, TB, ROWS / 2, C>(
+        tile_pairs<R, TB, ROWS / 2, GP, C>(
             s, T >> 1, [&](uint32_t j, uint32_t& i0, uint32_t& i1) { i0 = ins0_32(j, hb); i1 = i0 ^ lxy; },
             [&](uint32_t i0, uint32_t i1, cx<C>& v0, cx<C>& v1) {
                 const cx<C> f0 = (bp ^ par32(i0 & lyz)) ? fo : fe;
@@ -496,7 +529,7 @@ __device__ __forceinline__ void tile_apply(cx<R>* s, uint32_t T, uint64_t base, 
         const cx<C> f0 = ldc<C>(op.f, 0), f1 = ldc<C>(op.f, 1);
         const uint32_t bp = par64(base & op.gmask);
         const uint32_t lm = op.lmask;
-        tile_singles<R, TB, ROWS, C>(s, T, [&](uint32_t l, cx<C>& v) { v = cmul(v, (bp ^ par32(l & lm)) ? f1 : f0); });
+        tile_singles<R, TB, ROWS, 2 * GP, C>(s, T, [&](uint32_t l, cx<C>& v) { v = cmul(v, (bp ^ par32(l & lm)) ? f1 : f0); });
         return;
     }
     case OP_MATK: {
@@ -539,12 +572,26 @@ __device__ __forceinline__ void tile_apply(cx<R>* s, uint32_t T, uint64_t base, 
     case OP_DEPHASE1: {
         const C fac = (C)op.f[0];
         const uint32_t cq = op.p[0], ca = op.p[1];
+        if constexpr (GPC > 0) {
+            tile_singles<R, TB, ROWS, 2 * GPC, C>(s, T, [&](uint32_t l, cx<C>& v) {
+                if (chan_bit(l, base, cq) ^ chan_bit(l, base, ca)) v = cscale(v, fac);
+            });
+            return;
+        }
         for (uint32_t l = tid; l < T; l += TB)
             if (chan_bit(l, base, cq) ^ chan_bit(l, base, ca)) s[l] = dn<R>(cscale(up<C>(s[l]), fac));
         return;
     }
     case OP_DEPHASE2: {
         const C fac = (C)op.f[0];
+        if constexpr (GPC > 0) {
+            const uint32_t q0 = op.p[0], q1 = op.p[1], q2 = op.p[2], q3 = op.p[3];
+            tile_singles<R, TB, ROWS, 2 * GPC, C>(s, T, [&](uint32_t l, cx<C>& v) {
+                if ((chan_bit(l, base, q0) ^ chan_bit(l, base, q1)) | (chan_bit(l, base, q2) ^ chan_bit(l, base, q3)))
+                    v = cscale(v, fac);
+            });
+            return;
+        }
         for (uint32_t l = tid; l < T; l += TB) {
             uint32_t f = (chan_bit(l, base, op.p[0]) ^ chan_bit(l, base, op.p[1])) |
                          (chan_bit(l, base, op.p[2]) ^ chan_bit(l, base, op.p[3]));
@@ -555,6 +602,18 @@ __device__ __forceinline__ void tile_apply(cx<R>* s, uint32_t T, uint64_t base, 
     case OP_DEPOL1: {
         const C c1 = (C)op.f[0], c2 = (C)op.f[1], c3 = (C)op.f[2];
         const uint32_t bq = 1u << op.p[0], ba = 1u << op.p[1];
+        if constexpr (GPC / 2 > 0 && ROWS / 4 >= 2) {
+            if (T == (uint32_t)ROWS * TB) {
+                tile_quads<R, TB, ROWS / 4, GPC / 2, C>(s, T >> 2, bq, ba, [&](cx<C> (&v)[4]) {
+                    const cx<C> v00 = v[0], v11 = v[3];
+                    v[0] = mk<C>(c2 * v00.x + c1 * v11.x, c2 * v00.y + c1 * v11.y);
+                    v[1] = cscale(v[1], c3);
+                    v[2] = cscale(v[2], c3);
+                    v[3] = mk<C>(c1 * v00.x + c2 * v11.x, c1 * v00.y + c2 * v11.y);
+                });
+                return;
+            }
+        }
         for (uint32_t j = tid; j < (T >> 2); j += TB) {
             uint32_t i = ins0_mask32(j, bq | ba);
             cx<C> v00 = up<C>(s[i]), v01 = up<C>(s[i | bq]), v10 = up<C>(s[i | ba]), v11 = up<C>(s[i | bq | ba]);
@@ -568,6 +627,18 @@ __device__ __forceinline__ void tile_apply(cx<R>* s, uint32_t T, uint64_t base, 
     case OP_DAMP: {
         const C prob = (C)op.f[0], c1 = (C)op.f[1], c2 = (C)op.f[2];
         const uint32_t bq = 1u << op.p[0], ba = 1u << op.p[1];
+        if constexpr (GPC / 2 > 0 && ROWS / 4 >= 2) {
+            if (T == (uint32_t)ROWS * TB) {
+                tile_quads<R, TB, ROWS / 4, GPC / 2, C>(s, T >> 2, bq, ba, [&](cx<C> (&v)[4]) {
+                    const cx<C> v00 = v[0], v11 = v[3];
+                    v[0] = mk<C>(v00.x + prob * v11.x, v00.y + prob * v11.y);
+                    v[1] = cscale(v[1], c1);
+                    v[2] = cscale(v[2], c1);
+                    v[3] = cscale(v11, c2);
+                });
+                return;
+            }
+        }
         for (uint32_t j = tid; j < (T >> 2); j += TB) {
             uint32_t i = ins0_mask32(j, bq | ba);
             cx<C> v00 = up<C>(s[i]), v01 = up<C>(s[i | bq]), v10 = up<C>(s[i | ba]), v11 = up<C>(s[i | bq | ba]);
