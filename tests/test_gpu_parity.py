@@ -103,7 +103,11 @@ def test_sv_one_target_every_qubit_eager(fsa, orc, n):
 TUNINGS = {"default": {}, "valu-dense": {"FSA_MFMA": "0"}, "sweeps": {"FSA_SWEEPS": "1"},
            "register-tiles": {"FSA_RTILE": "1"}, "lds-tiles": {"FSA_RTILE": "0"},
            "sweeps-valu": {"FSA_SWEEPS": "1", "FSA_MFMA": "0"}, "tile13": {"FSA_TILE_BITS": "13"},
-           "tile13-low5": {"FSA_TILE_BITS": "13", "FSA_TILE_LOW": "5"}}
+           "tile13-low5": {"FSA_TILE_BITS": "13", "FSA_TILE_LOW": "5"},
+           # the plain greedy scheduler; 2^11 tiles (four workgroups per CU, two-pair
+           # batches); the fixed two workgroups per CU
+           "greedy-schedule": {"FSA_SCHED_SEEDS": "0"}, "tile11": {"FSA_TILE_BITS": "11"},
+           "fixed-grid": {"FSA_TILE_OCC": "0", "FSA_TILE_BITS": "11"}}
 
 
 @pytest.mark.parametrize("n", [4, 9, 12, 14, 18, 20])
