@@ -184,7 +184,7 @@ constexpr uint32_t MAX_STAGE_GRADS = 512;
 
 // the items of one stage on the LDS tile(s); OP_GRAD items add their tile sum
 // into gacc (tiles in a fixed order)
-template <typename R, int TB, bool REV>
+template <typename R, int TB, bool REV, int ROWS>
 __device__ __forceinline__ void run_items(cx<R>* phi, cx<R>* lam, uint32_t T, uint64_t base,
                                           const TileOp* __restrict__ items, uint32_t i0, uint32_t ni,
                                           const cx<double>* __restrict__ mats, const double* __restrict__ params,
@@ -212,8 +212,10 @@ __device__ __forceinline__ void run_items(cx<R>* phi, cx<R>* lam, uint32_t T, ui
             continue;
         }
         bind_param(op, params);
-        tile_apply<R, TB, 1, 1, double>(phi, T, base, op, mats);
-        if (REV) tile_apply<R, TB, 1, 1, double>(lam, T, base, op, mats);
+        // full tiles (T = ROWS * TB) load batches of four pairs before updating them
+        // (one workgroup per CU: registers to spare)
+        tile_apply<R, TB, 1, 1, double, ROWS, 4>(phi, T, base, op, mats);
+        if (REV) tile_apply<R, TB, 1, 1, double, ROWS, 4>(lam, T, base, op, mats);
         __syncthreads();
     }
 }
@@ -266,7 +268,7 @@ __global__ __launch_bounds__(TB) void k_batch(const BatchJob* __restrict__ jobs,
             for (uint32_t l = tid; l < T; l += TB) phi[l] = mk<R>(l == 0 ? (R)1 : (R)0, (R)0);
             __syncthreads();
             for (uint32_t s = J.fwd0; s < J.fwd0 + J.nfwd; s++)
-                run_items<R, TB, false>(phi, lam, T, 0, items, stages[s].item0, stages[s].nitems, mats, params, gacc,
+                run_items<R, TB, false, KMAX>(phi, lam, T, 0, items, stages[s].item0, stages[s].nitems, mats, params, gacc,
                                         red, true);
             double er = 0, ei = 0, nn = 0, nz = 0;
             apply_terms<R, TB>(phi, lam, 0, T, terms + J.term0, J.nterm, er, ei, nn);
@@ -280,7 +282,7 @@ __global__ __launch_bounds__(TB) void k_batch(const BatchJob* __restrict__ jobs,
             __syncthreads();
             for (uint32_t s = J.rev0; s < J.rev0 + J.nrev; s++) {
                 const BatchStage st = stages[s];
-                run_items<R, TB, true>(phi, lam, T, 0, items, st.item0, st.nitems, mats, params, gacc, red, true);
+                run_items<R, TB, true, KMAX>(phi, lam, T, 0, items, st.item0, st.nitems, mats, params, gacc, red, true);
                 flush_grads<TB>(st, items, gacc, res);
             }
             continue;
@@ -342,7 +344,7 @@ __global__ __launch_bounds__(TB) void k_batch(const BatchJob* __restrict__ jobs,
             if (1 < numTiles) load(1);
             for (uint64_t t = 0; t < numTiles; t++) {
                 const uint64_t base = pdep64(t, outMask);
-                run_items<R, TB, REV>(lds, lds + TMAX, T, base, items, st.item0, st.nitems, mats, params, gacc,
+                run_items<R, TB, REV, KMAX>(lds, lds + TMAX, T, base, items, st.item0, st.nitems, mats, params, gacc,
                                       red, t == 0);
                 __syncthreads();   // the last item may be a gradient sum still reading other rows
                 // Each thread writes its own rows of tile t back and fills them
