@@ -325,6 +325,15 @@ __global__ __launch_bounds__(TB, (sizeof(R) == 8 && ROWS >= 16) ? 2 : 4) void k_
     // MFMA kernels (twice the accumulators per thread) have no registers for it
     constexpr bool DRAIN = !(sizeof(R) == 4 && VAR >= 2);
     const unsigned nDrain = DRAIN ? (flags >> 8) & 0xffu : 0u;
+    // flags bit 6: the tiles are walked in bit-reversed index order, so the
+    // tiles in flight at once differ in the highest qubits outside the tile
+    // rather than the lowest (the HBM access pattern that measured faster)
+    const bool revOrder = (flags & 64u) != 0;
+    const uint32_t tileBits = 63u - (uint32_t)__builtin_clzll(numTiles | 1ULL);
+    auto tile_index = [&](uint64_t t) -> uint64_t {
+        if (revOrder && tileBits) return (uint64_t)__builtin_bitreverse64(t) >> (64u - tileBits);
+        return t;
+    };
     const unsigned nBody = numItems - nDrain;
 
     // Addresses are ORs of per-bit values selected by compile-time register
@@ -339,7 +348,7 @@ __global__ __launch_bounds__(TB, (sizeof(R) == 8 && ROWS >= 16) ? 2 : 4) void k_
 #define FSA_ROWADDR(r) ([&]() { uint64_t x_ = scLo; _Pragma("unroll") for (int j_ = 0; j_ < LOGROWS; j_++) if (((r) >> j_) & 1) x_ |= rowBit[j_]; return x_; }())
 #define FSA_PREFETCH(t)                                                                                 \
     do {                                                                                                \
-        const uint64_t pb_ = ins0_mask((t), geom.tileMask);                                             \
+        const uint64_t pb_ = ins0_mask(tile_index(t), geom.tileMask);                                   \
         _Pragma("unroll") for (int r = 0; r < ROWS; r++) {                                              \
             cx<R> v_ = a[pb_ | FSA_ROWADDR(r)]; bx[r] = v_.x; by[r] = v_.y;                             \
         }                                                                                               \
@@ -389,7 +398,7 @@ __global__ __launch_bounds__(TB, (sizeof(R) == 8 && ROWS >= 16) ? 2 : 4) void k_
     uint64_t next = tile + gridDim.x;
     if (next < numTiles) FSA_PREFETCH(next);
     for (;;) {
-        const uint64_t base = ins0_mask(tile, geom.tileMask);
+        const uint64_t base = ins0_mask(tile_index(tile), geom.tileMask);
         for (unsigned it = 0; it < nBody; it++) {
             const TileOp& op = items[it];
             if constexpr (SW) {
@@ -521,6 +530,10 @@ void launch_tile(hipStream_t s, cx<R>* a, unsigned nq, const TileGeom& geom, con
     const bool big = tileBytes > 64 * 1024;
     const uint32_t tb = big ? 1024u : 512u;
     const bool sw = (flags & 16u) != 0;
+    // tiles in bit-reversed order unless FSA_TILE_ORDER=0 (30-qubit 1000-gate
+    // circuit: 791-793 against 774 gates/s on one box, 777 against 760 on another)
+    const char* orderEnv = std::getenv("FSA_TILE_ORDER");
+    if (!(orderEnv && orderEnv[0] == '0')) flags |= 64u;
     auto* kern = k_tile<R, 512, 1, sweep_bits<R>(), 0, false>;
     if (big) {
         if constexpr (sizeof(R) == 8) {

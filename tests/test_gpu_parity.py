@@ -107,7 +107,9 @@ TUNINGS = {"default": {}, "valu-dense": {"FSA_MFMA": "0"}, "sweeps": {"FSA_SWEEP
            # the plain greedy scheduler; 2^11 tiles (four workgroups per CU, two-pair
            # batches); the fixed two workgroups per CU
            "greedy-schedule": {"FSA_SCHED_SEEDS": "0"}, "tile11": {"FSA_TILE_BITS": "11"},
-           "fixed-grid": {"FSA_TILE_OCC": "0", "FSA_TILE_BITS": "11"}}
+           "fixed-grid": {"FSA_TILE_OCC": "0", "FSA_TILE_BITS": "11"},
+           # tiles walked in index order instead of bit-reversed order
+           "natural-order": {"FSA_TILE_ORDER": "0"}}
 
 
 @pytest.mark.parametrize("n", [4, 9, 12, 14, 18, 20])
