@@ -327,11 +327,16 @@ __global__ __launch_bounds__(TB, (sizeof(R) == 8 && ROWS >= 16) ? 2 : 4) void k_
     const unsigned nDrain = DRAIN ? (flags >> 8) & 0xffu : 0u;
     // flags bit 6: the tiles are walked in bit-reversed index order, so the
     // tiles in flight at once differ in the highest qubits outside the tile
-    // rather than the lowest (the HBM access pattern that measured faster)
+    // rather than the lowest (the HBM access pattern that measured faster).
+    // FP64 only: in the FP32 plain kernel the same few scalar instructions per
+    // tile cost 13% (24-qubit passes, 335 against 297 us per launch) with
+    // identical vector code, registers and occupancy, so FP32 ignores the bit
     const bool revOrder = (flags & 64u) != 0;
     const uint32_t tileBits = 63u - (uint32_t)__builtin_clzll(numTiles | 1ULL);
     auto tile_index = [&](uint64_t t) -> uint64_t {
-        if (revOrder && tileBits) return (uint64_t)__builtin_bitreverse64(t) >> (64u - tileBits);
+        if constexpr (sizeof(R) == 8) {
+            if (revOrder && tileBits) return (uint64_t)__builtin_bitreverse64(t) >> (64u - tileBits);
+        }
         return t;
     };
     const unsigned nBody = numItems - nDrain;
@@ -530,8 +535,9 @@ void launch_tile(hipStream_t s, cx<R>* a, unsigned nq, const TileGeom& geom, con
     const bool big = tileBytes > 64 * 1024;
     const uint32_t tb = big ? 1024u : 512u;
     const bool sw = (flags & 16u) != 0;
-    // tiles in bit-reversed order unless FSA_TILE_ORDER=0 (30-qubit 1000-gate
-    // circuit: 791-793 against 774 gates/s on one box, 777 against 760 on another)
+    // FP64 tiles in bit-reversed order unless FSA_TILE_ORDER=0 (30-qubit 1000-gate
+    // circuit: 791-793 against 774 gates/s on one box, 777 against 760 on another);
+    // the FP32 kernels always walk the index order
     const char* orderEnv = std::getenv("FSA_TILE_ORDER");
     if (!(orderEnv && orderEnv[0] == '0')) flags |= 64u;
     auto* kern = k_tile<R, 512, 1, sweep_bits<R>(), 0, false>;
