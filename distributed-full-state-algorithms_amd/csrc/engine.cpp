@@ -717,12 +717,14 @@ TileOp translate_micro(const HOp& op, const int* pos, const int* sl, std::vector
     return t;
 }
 
-// A phase gadget or dephasing channel as an OP_DIAG4 item: amplitude l of a
-// tile with base index b is multiplied by f[(p1 << 1) | p2], p1 = parity of
-// (l & lmask) ^ parity of (b & gmask), p2 likewise with lmask2 and gq.  The
-// factors are the ones the op's own tile item multiplies by (1 where it
-// leaves an amplitude unchanged), so both forms give the same results.
-TileOp translate_diag4(const HOp& op, const int* pos, uint64_t tileMask) {
+// A dephasing channel, or one or two one-mask diagonals (phase gadgets, Pauli
+// gadgets and tensors of Z only; `second` may be null), as an OP_DIAG4 item:
+// amplitude l of a tile with base index b is multiplied by f[(p1 << 1) | p2],
+// p1 = parity of (l & lmask) ^ parity of (b & gmask), p2 likewise with lmask2
+// and gq.  The factors are the ones the ops' own tile items multiply by (1
+// where they leave an amplitude unchanged; two diagonals multiply), so both
+// forms give the same results up to rounding.
+TileOp translate_diag4(const HOp& op, const HOp* second, const int* pos, uint64_t tileMask) {
     TileOp t;
     std::memset(&t, 0, sizeof(t));
     t.kind = OP_DIAG4;
@@ -733,15 +735,33 @@ TileOp translate_diag4(const HOp& op, const int* pos, uint64_t tileMask) {
         out = gm & ~tileMask;
     };
     auto set = [&](int i, cd v) { t.f[2 * i] = v.real(); t.f[2 * i + 1] = v.imag(); };
+    // a one-mask diagonal: its parity mask and the factors on even / odd parity
+    auto one_mask = [](const HOp& d, uint64_t& mask, cd& even, cd& odd) {
+        if (d.kind == OP_PHASE) {          // e^{+i theta} on even parity, e^{-i theta} on odd
+            mask = d.pmask;
+            even = cd(std::cos(d.theta), std::sin(d.theta));
+            odd = cd(std::cos(d.theta), -std::sin(d.theta));
+        } else {                           // Z-only Pauli item: th + fe on even parity, th - fe on odd
+            mask = d.myz;
+            even = d.th + d.fe;
+            odd = d.th - d.fe;
+        }
+    };
     switch (op.kind) {
-    case OP_PHASE: {   // e^{+i theta} on even parity, e^{-i theta} on odd
-        split(op.pmask, t.lmask, t.gmask);
-        const cd e(std::cos(op.theta), std::sin(op.theta)), o(std::cos(op.theta), -std::sin(op.theta));
-        set(0, e); set(1, e); set(2, o); set(3, o);
+    case OP_PHASE:
+    case OP_PAULI: {   // mask 1 from op, mask 2 from the second op (factors 1 without one)
+        uint64_t m1 = 0, m2 = 0;
+        cd e1, o1, e2(1, 0), o2(1, 0);
+        one_mask(op, m1, e1, o1);
+        if (second) one_mask(*second, m2, e2, o2);
+        split(m1, t.lmask, t.gmask);
+        split(m2, t.lmask2, t.gq);
+        set(0, e1 * e2); set(1, e1 * o2); set(2, o1 * e2); set(3, o1 * o2);
         break;
     }
     case OP_DEPHASE1:   // scale where the bits of q and q + N differ
     case OP_DEPHASE2: { // ... where those of q1 / q1 + N or of q2 / q2 + N differ
+        if (second) fail(FSA_ERR_INTERNAL, "a dephasing channel fills a pass-end item alone");
         split((1ULL << op.q[0]) | (1ULL << op.q[1]), t.lmask, t.gmask);
         if (op.kind == OP_DEPHASE2) split((1ULL << op.q[2]) | (1ULL << op.q[3]), t.lmask2, t.gq);
         const cd one(1, 0), fac(op.kind == OP_DEPHASE2 ? 1.0 + op.c[0] : op.c[0], 0);   // the tile items' factors
@@ -860,10 +880,14 @@ void execute_impl(State& s, const std::vector<HOp>& ops, const std::vector<Stage
         std::vector<int> body, tail;
         // (not in FP32 MFMA passes: their kernels have no registers left for it)
         const bool drainDiag = env_flag("FSA_DRAIN_DIAG", true) && !(sizeof(R) == 4 && stageMfma);
+        // one pass-end item holds a dephasing channel or up to two one-mask
+        // diagonals (phase gadgets, Z-only Pauli gadgets and tensors)
+        auto one_mask = [](const HOp& d) { return d.kind == OP_PHASE || (d.kind == OP_PAULI && d.mxy == 0); };
         for (size_t i = 0; i < st.ops.size(); i++) {
             const HOp& d = ops[st.ops[i]];
-            bool drain = drainDiag && tail.size() < DRAIN_MAX &&
-                         (d.kind == OP_PHASE || d.kind == OP_DEPHASE1 || d.kind == OP_DEPHASE2);
+            bool drain = drainDiag &&
+                         (tail.empty() ? one_mask(d) || d.kind == OP_DEPHASE1 || d.kind == OP_DEPHASE2
+                                       : tail.size() < 2 * DRAIN_MAX && one_mask(d) && one_mask(ops[tail[0]]));
             for (size_t j = i + 1; drain && j < st.ops.size(); j++) drain = (ops[st.ops[j]].nd & d.touch) == 0;
             (drain ? tail : body).push_back(st.ops[i]);
         }
@@ -903,8 +927,9 @@ void execute_impl(State& s, const std::vector<HOp>& ops, const std::vector<Stage
             for (int oi : gr.ops) micro.push_back(translate_micro<R>(ops[oi], pos, sl, mats));
             items.push_back(it);
         }
-        for (int oi : tail) items.push_back(translate_diag4(ops[oi], pos, g.tileMask));
-        flags[si] |= (uint32_t)tail.size() << 8;
+        if (!tail.empty())
+            items.push_back(translate_diag4(ops[tail[0]], tail.size() > 1 ? &ops[tail[1]] : nullptr, pos, g.tileMask));
+        flags[si] |= (tail.empty() ? 0u : 1u) << 8;
         count[si] = items.size() - first[si];
         // every MFMA matrix of the pass goes to LDS (the left half of the real
         // form, 2 * 4^k reals)

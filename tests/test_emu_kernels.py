@@ -222,6 +222,25 @@ def test_phase_gadget_at_pass_end_on_cpu(drive, orc, tmp_path, drain):
     assert out["rel_err"] <= 1e-12 and out["passes"] == 1, out
 
 
+@pytest.mark.parametrize("drain", ["1", "0"])
+def test_z_pauli_pair_at_pass_end_on_cpu(drive, orc, tmp_path, drain):
+    # Z-only Pauli gadgets and tensors are one-mask diagonals too: two of the
+    # three diagonals that commute to the pass end share its register item
+    # (factors multiplied), the third stays an LDS item; one with an X does not
+    # commute past the gate on its qubit and stays in place
+    rng = np.random.default_rng(36)
+    n = 15
+    ops = [("oneTargGate", (q, _unitary(rng, 2))) for q in (3, 9, 12, 14)]
+    ops += [("pauliGadget", ([1, 6, 13], [3, 3, 3], 0.4)), ("oneTargGate", (7, _unitary(rng, 2))),
+            ("pauliTensor", ([0, 10], [3, 3])), ("pauliGadget", ([4, 7], [1, 3], 0.9)),
+            ("oneTargGate", (4, _unitary(rng, 2))), ("phaseGadget", ([2, 9, 14], -1.1)),
+            ("oneTargGate", (11, _unitary(rng, 2)))]
+    path = tmp_path / "drainz.bin"
+    _circuit_file(path, orc, n, ops, rng)
+    out = _run(drive, ["circuit", path], {"FSA_DRAIN_DIAG": drain})
+    assert out["rel_err"] <= 1e-12 and out["passes"] == 1, out
+
+
 def test_mixed_circuit_on_13_bit_tiles_on_cpu(drive, orc, tmp_path):
     rng = np.random.default_rng(32)
     n = 15
