@@ -168,6 +168,16 @@ __device__ __forceinline__ void adj_st(cx<R>* p, cx<R> v) {
     else *p = v;
 }
 
+// First block of a sweep kernel's grid-stride loop.  FP32 sweeps run 4-5%
+// faster with each XCD on a contiguous eighth of the blocks (stream_block);
+// FP64 ones 4-9% slower (24-qubit adjoint, same box), so they keep the
+// round-robin order.
+template <typename R>
+__device__ __forceinline__ uint64_t adjoint_block() {
+    if constexpr (sizeof(R) == 4) return stream_block();
+    return blockIdx.x;
+}
+
 template <typename R, bool NT>
 __global__ __launch_bounds__(BLOCK) void k_adjoint_step(cx<R>* __restrict__ phi, cx<R>* __restrict__ lam,
                                                         uint64_t numIts, unsigned hb, AdjointStepDev st,
@@ -176,7 +186,7 @@ __global__ __launch_bounds__(BLOCK) void k_adjoint_step(cx<R>* __restrict__ phi,
     const cx<R> fo = mk<R>(-fe.x, -fe.y);
     const cx<R> e0 = mk<R>((R)st.e0r, (R)st.e0i), e1 = mk<R>((R)st.e1r, (R)st.e1i);
     double vr = 0, vi = 0;
-    for (uint64_t k = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; k < numIts; k += (uint64_t)gridDim.x * BLOCK) {
+    for (uint64_t k = adjoint_block<R>() * BLOCK + threadIdx.x; k < numIts; k += (uint64_t)gridDim.x * BLOCK) {
         if (st.mxy) {
             const uint64_t j0 = ins0(k, hb), j1 = j0 ^ st.mxy;
             const cx<R> p0 = adj_ld<R, NT>(phi + j0), p1 = adj_ld<R, NT>(phi + j1);
@@ -241,7 +251,7 @@ __global__ __launch_bounds__(BLOCK) void k_adjoint_diag(cx<R>* __restrict__ phi,
     double vr[ADJ_DIAG_MAX], vi[ADJ_DIAG_MAX];
 #pragma unroll
     for (unsigned g = 0; g < ADJ_DIAG_MAX; g++) { vr[g] = 0; vi[g] = 0; }
-    for (uint64_t k = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; k < num; k += (uint64_t)gridDim.x * BLOCK) {
+    for (uint64_t k = adjoint_block<R>() * BLOCK + threadIdx.x; k < num; k += (uint64_t)gridDim.x * BLOCK) {
         const cx<R> p = adj_ld<R, true>(phi + k), l = adj_ld<R, true>(lam + k);
         const double wr = (double)l.x * (double)p.x + (double)l.y * (double)p.y;
         const double wi = (double)l.x * (double)p.y - (double)l.y * (double)p.x;
@@ -296,7 +306,7 @@ __global__ __launch_bounds__(BLOCK) void k_adjoint_qrun(cx<R>* __restrict__ phi,
 #pragma unroll
         for (int g = 0; g < G; g++) if ((e >> g) & 1u) off[e] |= 1ULL << st.q[g];
     }
-    for (uint64_t t = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; t < numGroups; t += (uint64_t)gridDim.x * BLOCK) {
+    for (uint64_t t = adjoint_block<R>() * BLOCK + threadIdx.x; t < numGroups; t += (uint64_t)gridDim.x * BLOCK) {
         uint64_t x = t;
 #pragma unroll
         for (int i = 0; i < G; i++) x = ins0(x, st.sorted[i]);
@@ -384,7 +394,7 @@ __global__ __launch_bounds__(BLOCK) void k_adjoint_qrun_lane(cx<R>* __restrict__
 #pragma unroll
     for (int g = 0; g < DM; g++) { dr[g] = 0; di[g] = 0; }
     // num is a multiple of 64 and so is the stride: a wave's lanes stay together
-    for (uint64_t j = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; j < num; j += (uint64_t)gridDim.x * BLOCK) {
+    for (uint64_t j = adjoint_block<R>() * BLOCK + threadIdx.x; j < num; j += (uint64_t)gridDim.x * BLOCK) {
         cx<R> p = adj_ld<R, true>(phi + j), l = adj_ld<R, true>(lam + j);
 #pragma unroll
         for (int g = 0; g < G; g++) {

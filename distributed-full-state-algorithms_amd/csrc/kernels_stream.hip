@@ -43,7 +43,7 @@ inline unsigned blocks_for(uint64_t numItems, int itemsPerThread) {
 template <typename R, bool NT>
 __global__ __launch_bounds__(BLOCK) void k_mat1(cx<R>* __restrict__ a, uint64_t numIts, unsigned t,
                                                 uint64_t qmask, uint64_t cmask, Mat2<R> g) {
-    const uint64_t first = (uint64_t)blockIdx.x * (BLOCK * ITEMS) + threadIdx.x;
+    const uint64_t first = stream_block() * (BLOCK * ITEMS) + threadIdx.x;
     const uint64_t tb = 1ULL << t;
     cx<R> v0[ITEMS], v1[ITEMS];
     uint64_t idx[ITEMS];
@@ -70,7 +70,7 @@ __global__ __launch_bounds__(BLOCK) void k_mat1(cx<R>* __restrict__ a, uint64_t 
 // target 0: 6.07 ms against 6.63 ms for k_mat1's cached form.
 template <typename R>
 __global__ __launch_bounds__(BLOCK) void k_mat1_lane(cx<R>* __restrict__ a, uint64_t num, unsigned t, Mat2<R> g) {
-    const uint64_t first = (uint64_t)blockIdx.x * (BLOCK * ITEMS) + threadIdx.x;
+    const uint64_t first = stream_block() * (BLOCK * ITEMS) + threadIdx.x;
     const int m = 1 << t;
     cx<R> v[ITEMS];
 #pragma unroll
@@ -94,7 +94,7 @@ __global__ __launch_bounds__(BLOCK) void k_mat1_lane(cx<R>* __restrict__ a, uint
 template <typename R, bool NT>
 __global__ __launch_bounds__(BLOCK) void k_swap(cx<R>* __restrict__ a, uint64_t numIts, uint64_t b1,
                                                 uint64_t b2) {
-    const uint64_t first = (uint64_t)blockIdx.x * (BLOCK * ITEMS) + threadIdx.x;
+    const uint64_t first = stream_block() * (BLOCK * ITEMS) + threadIdx.x;
     const uint64_t m = b1 | b2;
     R ax[ITEMS], ay[ITEMS], bx[ITEMS], by[ITEMS];    // split re/im keeps them in registers
     uint64_t idx[ITEMS];
@@ -120,7 +120,7 @@ __global__ __launch_bounds__(BLOCK) void k_swap(cx<R>* __restrict__ a, uint64_t 
 template <typename R, bool NT>
 __global__ __launch_bounds__(BLOCK) void k_pauli(cx<R>* __restrict__ a, uint64_t numIts, unsigned hb,
                                                  uint64_t mxy, uint64_t myz, cx<R> th, cx<R> fe) {
-    const uint64_t first = (uint64_t)blockIdx.x * (BLOCK * ITEMS) + threadIdx.x;
+    const uint64_t first = stream_block() * (BLOCK * ITEMS) + threadIdx.x;
     const cx<R> fo = mk<R>(-fe.x, -fe.y);
     cx<R> v0[ITEMS], v1[ITEMS];
     uint64_t idx[ITEMS];
@@ -147,7 +147,7 @@ __global__ __launch_bounds__(BLOCK) void k_pauli(cx<R>* __restrict__ a, uint64_t
 template <typename R>
 __global__ __launch_bounds__(BLOCK) void k_pauli_diag(cx<R>* __restrict__ a, uint64_t num, uint64_t myz,
                                                       cx<R> th, cx<R> fe) {
-    const uint64_t first = (uint64_t)blockIdx.x * (BLOCK * ITEMS) + threadIdx.x;
+    const uint64_t first = stream_block() * (BLOCK * ITEMS) + threadIdx.x;
     const cx<R> fo = mk<R>(-fe.x, -fe.y);
     cx<R> v[ITEMS];
 #pragma unroll
@@ -169,7 +169,7 @@ __global__ __launch_bounds__(BLOCK) void k_pauli_diag(cx<R>* __restrict__ a, uin
 template <typename R>
 __global__ __launch_bounds__(BLOCK) void k_phase(cx<R>* __restrict__ a, uint64_t num, uint64_t mask,
                                                  cx<R> f0, cx<R> f1) {
-    const uint64_t first = (uint64_t)blockIdx.x * (BLOCK * ITEMS) + threadIdx.x;
+    const uint64_t first = stream_block() * (BLOCK * ITEMS) + threadIdx.x;
     cx<R> v[ITEMS];
 #pragma unroll
     for (int p = 0; p < ITEMS; p++) {
@@ -185,7 +185,7 @@ __global__ __launch_bounds__(BLOCK) void k_phase(cx<R>* __restrict__ a, uint64_t
 
 template <typename R>
 __global__ __launch_bounds__(BLOCK) void k_scale(cx<R>* __restrict__ a, uint64_t num, R f) {
-    const uint64_t first = (uint64_t)blockIdx.x * (BLOCK * ITEMS) + threadIdx.x;
+    const uint64_t first = stream_block() * (BLOCK * ITEMS) + threadIdx.x;
 #pragma unroll
     for (int p = 0; p < ITEMS; p++) {
         uint64_t j = first + (uint64_t)p * BLOCK;
@@ -203,7 +203,7 @@ __global__ __launch_bounds__(BLOCK) void k_matk_reg(cx<R>* __restrict__ a, uint6
     __shared__ cx<R> sm[D * D];
     for (int i = threadIdx.x; i < D * D; i += BLOCK) sm[i] = mat[i];
     __syncthreads();
-    const uint64_t g = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+    const uint64_t g = stream_block() * BLOCK + threadIdx.x;
     if (g >= numGroups) return;
     const uint64_t base = ins0_mask(g, tmask);
     cx<R> v[D];
@@ -243,7 +243,7 @@ __global__ __launch_bounds__(BLOCK) void k_matk_lds(cx<R>* __restrict__ a, unsig
 template <typename R>
 __global__ __launch_bounds__(BLOCK) void k_dephase1(cx<R>* __restrict__ a, uint64_t numIts, unsigned q,
                                                     unsigned alt, R fac) {
-    const uint64_t first = (uint64_t)blockIdx.x * (BLOCK * ITEMS) + threadIdx.x;
+    const uint64_t first = stream_block() * (BLOCK * ITEMS) + threadIdx.x;
     const uint64_t m = (1ULL << q) | (1ULL << alt);
 #pragma unroll
     for (int p = 0; p < ITEMS; p++) {
@@ -262,7 +262,7 @@ __global__ __launch_bounds__(BLOCK) void k_dephase1(cx<R>* __restrict__ a, uint6
 template <typename R>
 __global__ __launch_bounds__(BLOCK) void k_dephase2(cx<R>* __restrict__ a, uint64_t num, unsigned q1,
                                                     unsigned a1, unsigned q2, unsigned a2, R fac) {
-    const uint64_t first = (uint64_t)blockIdx.x * (BLOCK * ITEMS) + threadIdx.x;
+    const uint64_t first = stream_block() * (BLOCK * ITEMS) + threadIdx.x;
 #pragma unroll
     for (int p = 0; p < ITEMS; p++) {
         uint64_t j = first + (uint64_t)p * BLOCK;
@@ -277,7 +277,7 @@ __global__ __launch_bounds__(BLOCK) void k_dephase2(cx<R>* __restrict__ a, uint6
 template <typename R>
 __global__ __launch_bounds__(BLOCK) void k_depol1(cx<R>* __restrict__ a, uint64_t numIts, unsigned q,
                                                   unsigned alt, R c1, R c2, R c3) {
-    const uint64_t first = (uint64_t)blockIdx.x * (BLOCK * ITEMS) + threadIdx.x;
+    const uint64_t first = stream_block() * (BLOCK * ITEMS) + threadIdx.x;
     const uint64_t bq = 1ULL << q, ba = 1ULL << alt;
 #pragma unroll
     for (int p = 0; p < ITEMS; p++) {
@@ -301,7 +301,7 @@ template <typename R>
 __global__ __launch_bounds__(BLOCK) void k_depol2(cx<R>* __restrict__ a, uint64_t numIts, uint64_t mask,
                                                   uint64_t b0, uint64_t b1, uint64_t b2, uint64_t b3,
                                                   R c1, R c2, R c3) {
-    const uint64_t k = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+    const uint64_t k = stream_block() * BLOCK + threadIdx.x;
     if (k >= numIts) return;
     const uint64_t base = ins0_mask(k, mask);
     const uint64_t bit[4] = {b0, b1, b2, b3};   // q1, q2, q1+N, q2+N
@@ -331,7 +331,7 @@ __global__ __launch_bounds__(BLOCK) void k_depol2(cx<R>* __restrict__ a, uint64_
 template <typename R>
 __global__ __launch_bounds__(BLOCK) void k_damp(cx<R>* __restrict__ a, uint64_t numIts, unsigned q,
                                                 unsigned alt, R prob, R c1, R c2) {
-    const uint64_t first = (uint64_t)blockIdx.x * (BLOCK * ITEMS) + threadIdx.x;
+    const uint64_t first = stream_block() * (BLOCK * ITEMS) + threadIdx.x;
     const uint64_t bq = 1ULL << q, ba = 1ULL << alt;
 #pragma unroll
     for (int p = 0; p < ITEMS; p++) {

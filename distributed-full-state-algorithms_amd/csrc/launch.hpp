@@ -10,6 +10,21 @@ namespace fsa {
 
 constexpr int BLOCK = 256;
 
+// The block's share of the work (streaming kernels, and the grid-stride
+// loops of the FP32 adjoint sweep kernels).  Blocks are dispatched round-robin
+// over the 8 XCDs (block b to XCD b % 8); this order gives each XCD a contiguous eighth
+// of the shares instead of every eighth one.  At 30 qubits FP64 it streams
+// 3-8% faster for most targets (phase gadget 5.69 -> 6.05 TB/s, swap 5.1-5.6
+// -> 5.9-6.0, one-qubit gate on qubits 3-10 5.4 -> 5.8) and 1-4% slower for a
+// few (one-qubit gate on qubit 18, 20, 24 or 28).  Every block index still
+// maps to one share, so the kernels' bounds checks are unchanged; per-block
+// partial sums keep the physical block's slot.
+__device__ __forceinline__ uint64_t stream_block() {
+    const uint32_t G = gridDim.x;
+    if (G % 8u) return blockIdx.x;
+    return (uint64_t)(blockIdx.x % 8u) * (G / 8u) + blockIdx.x / 8u;
+}
+
 template <typename R> struct Offs32 { uint64_t o[32]; };
 
 // ---- streaming kernels: one HBM pass per operation ---------------------------
