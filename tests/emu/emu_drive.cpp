@@ -1,7 +1,9 @@
 // Drives libfsa's real planner and kernel code on the CPU stand-in.
 //
 //   emu_drive circuit <file>              fused circuit (make_plan + execute: k_tile, streaming kernels);
-//                                         EMU_EAGER=1 runs op by op instead, EMU_FP32=1 on an FP32 state
+//                                         EMU_EAGER=1 runs the ops through execute_now (as one list),
+//                                         EMU_EAGER=2 one execute_now per op (each op alone: its
+//                                         streaming kernel), EMU_FP32=1 on an FP32 state
 //   emu_drive batch <file> TB BITS LOW P  batch (fsa_batch_* + k_batch), P = 0 FP64 / 1 FP32
 //   emu_drive adjoint <file> P            fsa_adjoint_gradient from |0..0> (fused forward pass,
 //                                         reverse-sweep kernels of kernels_reduce.hip)
@@ -79,8 +81,12 @@ int run_circuit(const char* path) {
     size_t tiled = 0;
     for (const Stage& st : p.stages) tiled += st.tiled;
     profile_enable(true);
-    if (eager) execute_now(s, c->c.ops);
-    else execute(s, p);
+    if (eager && std::getenv("EMU_EAGER")[0] == '2') {
+        for (const HOp& op : c->c.ops) execute_now(s, std::vector<HOp>{op});
+    } else if (eager) {
+        execute_now(s, c->c.ops);
+    }
+    if (!eager) execute(s, p);
     // launches per kernel family (the profiler's names), e.g. tile_pass_mfma=3
     for (const ProfEntry& e : profile_read()) std::printf("%s=%llu ", e.name.c_str(), (unsigned long long)e.count);
     profile_enable(false);

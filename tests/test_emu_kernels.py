@@ -20,7 +20,9 @@ compared with the oracle:
     FP32, against an adjoint computed with the oracle;
   * the single-state adjoint sweep (kernels_reduce.hip): runs of one-qubit X/Y
     gadgets (group and lane forms, a diagonal tail), runs of Z-only gadgets and
-    single steps, FP64 and FP32, against the same oracle adjoint.
+    single steps, FP64 and FP32, against the same oracle adjoint;
+  * every op kind as its own streaming kernel, on grids that take the
+    XCD-contiguous block order and on grids that keep the round-robin one.
 
 Skipped when the ROCm clang++ is not installed.
 """
@@ -380,6 +382,21 @@ def test_dense_gate_gemm_on_cpu(drive, orc, tmp_path, k, n, eager, fp32, g3):
         env["EMU_FP32"] = "1"
     out = _run(drive, ["circuit", path], env)
     assert out["rel_err"] <= (2e-6 if fp32 else 1e-12), out
+
+
+@pytest.mark.parametrize("n", [12, 15])
+def test_eager_streaming_kernels_on_cpu(drive, orc, tmp_path, n):
+    # every op as its own streaming kernel (kernels_stream.hip).  At 15 qubits
+    # most grids are multiples of 8 blocks, which take the XCD-contiguous block
+    # order; at 12 qubits (and for 2-control gates at 15) they have 1-4 blocks
+    # and keep the round-robin one.  Both must cover every item exactly once.
+    rng = np.random.default_rng(90 + n)
+    ops = [_random_op(rng, n) for _ in range(40)]
+    path = tmp_path / "eager.bin"
+    _circuit_file(path, orc, n, ops, rng)
+    out = _run(drive, ["circuit", path], {"EMU_EAGER": "2"})
+    assert out["rel_err"] <= 1e-12, out
+    assert out.get("mat1", 0) > 0 and out.get("pauli", 0) + out.get("phase", 0) > 0, out
 
 
 # ---- register-resident passes (kernels_rtile.hip, rtile_plan.cpp) ---------------
