@@ -14,7 +14,10 @@ namespace fsa {
 namespace {
 
 constexpr int RED_ITEMS = 8;
-constexpr unsigned RED_MAX_BLOCKS = 2048;
+// grid cap of the grid-stride reductions and adjoint sweep kernels: four
+// 256-thread blocks per CU.  The 24-qubit adjoint ran 1.3-1.4% faster than with
+// 2048 (FP64 diagonal passes 4% faster) and 4096 was slower (same box).
+constexpr unsigned RED_MAX_BLOCKS = 1024;
 
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
